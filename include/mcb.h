@@ -1,0 +1,204 @@
+/*
+ * libmcb — MI355X-native D3Q19 phase-field bubble lattice-Boltzmann step.
+ *
+ * C ABI (plain pointers, sizes and status codes; no HIP or torch types).
+ * It replaces the reference's time step and driver surface
+ * (kreeyto/MULTIC-BUBBLE-CUDA):
+ *   - global device buffers + initializeVars()      src/var.cu:13-18, 57-114
+ *   - init kernels launched by main()              src/main.cu:56-68, src/init.cu:3-80
+ *   - the time loop body (10 launches, 3 syncs)    src/main.cu:74-182
+ *   - copyAndSaveToBinary / generateSimulationInfoFile
+ *                                                  src/auxFunctions.cu:12-67
+ *   - freeMemory                                   src/auxFunctions.cu:4-10
+ * The per-kernel operator face (src/kernels.cuh) is in mcb_ops.h.
+ *
+ * Ownership: the library owns device memory of a context; the caller owns
+ * every host buffer. Errors: every call returns mcb_status and never aborts;
+ * mcb_last_error() holds a message for the last failing call on a context.
+ * Threading: a context is used by one host thread at a time; all its work is
+ * ordered on the context's stream; read calls synchronise that stream.
+ */
+#ifndef MCB_H
+#define MCB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCB_ABI_VERSION 1
+
+typedef enum mcb_status {
+    MCB_OK = 0,
+    MCB_ERR_INVALID_ARG = 1,   /* bad parameter, size or pointer */
+    MCB_ERR_NO_DEVICE = 2,     /* no usable gfx950 device */
+    MCB_ERR_OUT_OF_MEMORY = 3, /* device allocation failed */
+    MCB_ERR_HIP = 4,           /* HIP runtime or launch error */
+    MCB_ERR_UNSUPPORTED = 5,   /* combination not implemented */
+    MCB_ERR_IO = 6,            /* file could not be written */
+    MCB_ERR_NONFINITE = 7,     /* NaN/Inf detected by a diagnostic */
+    MCB_ERR_CAPACITY = 8       /* admission would exceed a memory budget */
+} mcb_status;
+
+/* Boundary handling. REFERENCE reproduces src/lbm.cu (interior-only
+ * updates, ghost-layer fgBoundary + zero-gradient phi); PERIODIC updates
+ * every cell with periodic wrap (the benchmark cases). */
+typedef enum mcb_bc_mode { MCB_BC_PERIODIC = 0, MCB_BC_REFERENCE = 1 } mcb_bc_mode;
+
+/* Lattice weight table. LITERAL = the reference's float literals
+ * (src/var.cu:40-45, sum 1+2^-26); UNIT_SUM = nearest floats with exact unit
+ * sum (removes a +1.49e-8/step mass gain). */
+typedef enum mcb_weight_set { MCB_WEIGHTS_LITERAL = 0, MCB_WEIGHTS_UNIT_SUM = 1 } mcb_weight_set;
+
+/* Population storage precision; arithmetic is FP32 in every form. */
+typedef enum mcb_storage { MCB_STORE_FP32 = 0, MCB_STORE_FP16 = 1, MCB_STORE_BF16 = 2 } mcb_storage;
+
+/* Kernel path. FUSED = 2 passes per step (phase recovery, then
+ * normals/curvature/moments/collisions fused with LDS-staged stencils);
+ * STAGED = the reference's per-kernel decomposition (mcb_ops.h kernels). */
+typedef enum mcb_path { MCB_PATH_AUTO = 0, MCB_PATH_FUSED = 1, MCB_PATH_STAGED = 2 } mcb_path;
+
+typedef struct mcb_params {
+    int nx, ny, nz;     /* lattice sides (>= 3; reference 64 x 64 x 256, src/var.cu:3-4) */
+    int bc_mode;        /* mcb_bc_mode */
+    int weight_set;     /* mcb_weight_set */
+    int storage;        /* mcb_storage */
+    int path;           /* mcb_path */
+    int device;         /* HIP device ordinal */
+    float tau_f;        /* fluid relaxation time H_TAU (src/var.cu:21), > 0.5 */
+    float tau_g;        /* phase relaxation time; 1 = the reference collisionPhase */
+    float sigma;        /* surface tension H_SIGMA (src/var.cu:25) */
+    float sharp_c;      /* interface sharpening H_SHARP_C; < 0 selects 0.15f*3.0f (src/var.cu:24) */
+    int moving_wall;    /* 1: also run the reference's movingWallBC (dead code, src/lbm.cu:430-461) */
+    int reserved[7];
+} mcb_params;
+
+typedef struct mcb_bubble {
+    float cx, cy, cz;   /* centre in cell-index coordinates (reference: N*0.5f, src/init.cu:41-43) */
+    float r;            /* radius (reference 20.0f, src/init.cu:39) */
+} mcb_bubble;
+
+/* Readable/writable fields. PHI is the phase buffer as the reference keeps
+ * d_phi: after mcb_step it holds the phi computed at the start of the last
+ * iteration (REFERENCE mode: with boundaryConditions applied). RHO, UX, UY,
+ * UZ, FX, FY, FZ, NORMX, NORMY, NORMZ hold the values of the most recent
+ * mcb_compute_macros call. F and G (19 planes, population-major, x fastest)
+ * are the pre-collision populations in the reference layout (src/header.cuh:21). */
+typedef enum mcb_field {
+    MCB_FIELD_PHI = 0, MCB_FIELD_RHO = 1, MCB_FIELD_UX = 2, MCB_FIELD_UY = 3, MCB_FIELD_UZ = 4,
+    MCB_FIELD_FX = 5, MCB_FIELD_FY = 6, MCB_FIELD_FZ = 7,
+    MCB_FIELD_NORMX = 8, MCB_FIELD_NORMY = 9, MCB_FIELD_NORMZ = 10,
+    MCB_FIELD_F = 16, MCB_FIELD_G = 17
+} mcb_field;
+
+typedef struct mcb_ctx mcb_ctx;
+
+/* Reference case constants (src/var.cu:3-4, 21-25; src/init.cu:39). */
+mcb_status mcb_params_reference(mcb_params* p);
+/* Check a parameter set without touching a device. */
+mcb_status mcb_validate_params(const mcb_params* p);
+/* Device bytes a context with these parameters allocates. */
+size_t mcb_footprint_bytes(const mcb_params* p);
+
+/* Allocate a context on p->device. stream: a hipStream_t or NULL (the
+ * library then creates its own). Replaces initializeVars (src/var.cu:57-114). */
+mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out);
+mcb_status mcb_destroy(mcb_ctx* ctx);
+
+/* Initial state of src/main.cu:56-68: rho = 1, u = 0, f and g at
+ * equilibrium, phi = max over bubbles of 0.5+0.5*tanh(2(R-r)/width)
+ * (src/init.cu:39-46; width 3 in the reference). PERIODIC mode measures
+ * minimum-image distances; REFERENCE mode leaves the faces at phi = 0. */
+mcb_status mcb_init_bubbles(mcb_ctx* ctx, int n, const mcb_bubble* bubbles, float width);
+/* Same, from an arbitrary host phi field of nx*ny*nz floats. */
+mcb_status mcb_init_phi(mcb_ctx* ctx, const float* phi_host, size_t n);
+
+/* Advance nsteps full iterations of the reference loop body. */
+mcb_status mcb_step(mcb_ctx* ctx, int nsteps);
+/* Advance nsteps and report the device time of those steps (hipEvents). */
+mcb_status mcb_step_timed(mcb_ctx* ctx, int nsteps, double* ms);
+/* Observables of the current state (phases phiCalc..momentiCalc of the
+ * next iteration, without advancing): fills PHI, RHO, U*, F*, NORM*. */
+mcb_status mcb_compute_macros(mcb_ctx* ctx);
+
+mcb_status mcb_read_field(mcb_ctx* ctx, int field, float* host, size_t n);
+mcb_status mcb_write_field(mcb_ctx* ctx, int field, const float* host, size_t n);
+
+/* FP64 diagnostics over the fields of the last mcb_compute_macros:
+ * sum of phi and kinetic energy sum(0.5*rho*|u|^2), reduced on the device
+ * with a fixed-order tree. Returns MCB_ERR_NONFINITE if either is not finite. */
+mcb_status mcb_reduce(mcb_ctx* ctx, double* sum_phi, double* kinetic);
+
+mcb_status mcb_synchronize(mcb_ctx* ctx);
+
+/* Device time of each pass of the fused step, summed over nsteps steps
+ * (hipEvents on the context stream around each launch):
+ * out_ms[0] = phase recovery (k_phase), out_ms[1] = collision (k_collide).
+ * The steps advance the state. No counterpart in the reference (its only
+ * timer is a whole-program wall clock, src/main.cu:203-211). */
+mcb_status mcb_step_kernel_times(mcb_ctx* ctx, int nsteps, double* out_ms, int n_out);
+
+/* Tile of the fused collision pass: tx x ty cells per workgroup marching
+ * zc planes; (tx, ty) must be one of mcb_tiling_at(0..mcb_tiling_count()-1). */
+mcb_status mcb_set_tiling(mcb_ctx* ctx, int tx, int ty, int zc);
+mcb_status mcb_get_tiling(const mcb_ctx* ctx, int* tx, int* ty, int* zc);
+int mcb_tiling_count(void);
+mcb_status mcb_tiling_at(int i, int* tx, int* ty);
+
+int mcb_abi_version(void);
+const char* mcb_last_error(const mcb_ctx* ctx);
+const char* mcb_status_string(mcb_status s);
+/* Parameters the context was created with. */
+mcb_status mcb_get_params(const mcb_ctx* ctx, mcb_params* out);
+
+/* ---- Host-side helpers (no device needed) ---- */
+
+/* A complete case: parameters, initial bubbles, step count and save cadence. */
+#define MCB_MAX_BUBBLES 8
+typedef struct mcb_case {
+    mcb_params params;
+    int n_bubbles;
+    mcb_bubble bubbles[MCB_MAX_BUBBLES];
+    float width;        /* tanh profile width (3 in every preset) */
+    int nsteps;         /* NSTEPS (src/main.cu:35); the loop runs NSTEPS+1 iterations */
+    int save_every;     /* MACRO_SAVE (src/main.cu:35) */
+} mcb_case;
+
+/* Named presets. "reference": src/var.cu + src/init.cu + src/main.cu
+ * (64x64x256, ghost-layer boundaries, tau 0.505, one R=20 bubble at the
+ * centre, 25000 steps, phi saved every 100). "cfg0" .. "cfg2": the periodic
+ * benchmark cases (64^3 R=10 centred; 128^3 seeded R in [12,24] off-centre;
+ * 256^3 two seeded non-overlapping bubbles R in [16,32]). Random draws use
+ * splitmix64(seed) mapped to [0,1) by the top 53 bits, so every consumer of
+ * a (name, seed) pair gets the same geometry. */
+mcb_status mcb_case_preset(const char* name, uint64_t seed, mcb_case* out);
+
+/* The initial phi of mcb_init_bubbles, on the host (nx*ny*nz floats). */
+mcb_status mcb_host_init_phi(const mcb_params* p, int n, const mcb_bubble* bubbles, float width,
+                             float* phi_out, size_t len);
+
+/* <dir>/<id>_info.txt in the reference's format (src/auxFunctions.cu:24-40).
+ * mlups <= 0 writes the reference's literal 1.187970e+01. */
+mcb_status mcb_write_info_file(const char* path, const char* sim_id, const char* velocity_set,
+                               int nx, int ny, int nz, int save_steps, int nsteps, float tau, double mlups);
+/* <dir><id>_<var><step:06d>.bin, raw float32 x-fastest (src/auxFunctions.cu:49-67). */
+mcb_status mcb_write_field_file(const char* dir, const char* sim_id, const char* var_name, int step,
+                                const float* data, size_t n);
+
+/* Device helpers for callers that drive mcb_ops.h on their own buffers. */
+mcb_status mcb_device_count(int* count);
+mcb_status mcb_device_alloc(int device, size_t bytes, void** dptr);
+mcb_status mcb_device_free(void* dptr);
+mcb_status mcb_memcpy_htod(void* dst, const void* src, size_t bytes);
+mcb_status mcb_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+mcb_status mcb_device_synchronize(void);
+/* Time of one plain float4 device-to-device copy of `bytes` (mean of iters),
+ * the achievable-bandwidth yardstick for the roofline. */
+mcb_status mcb_bench_copy(int device, size_t bytes, int iters, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCB_H */

@@ -1,0 +1,523 @@
+// TEST INFRASTRUCTURE — NOT PART OF THE PRODUCT.
+//
+// Serial C++ transcription of the reference D3Q19 phase-field bubble LBM
+// (kreeyto/MULTIC-BUBBLE-CUDA, src/lbm.cu, src/init.cu, src/var.cu, src/main.cu).
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it, and only as the checker / CPU baseline. libmcb never links it.
+//
+// Every stage is one loop nest over cells, in the reference's program order
+// (src/main.cu:74-190), with the reference's expression forms kept literally:
+// int link tables converted to float inside the link loops, `3.0f*W[l]`
+// coefficients, `*invCssq` in momentiCalc versus `/CSSQ` in collisionFluid,
+// and the same left-to-right summation orders. Two deliberate, documented
+// differences from the CUDA source:
+//   * rsqrtf(x) (lbm.cu:66) is written 1/sqrt(x): rsqrtf is an approximate
+//     instruction whose bits are implementation-defined; 1/sqrt is IEEE.
+//   * the power-of-two wrap `& (N-1)` (lbm.cu:350-352) is a true modulo, equal
+//     for power-of-two sides and correct for all others.
+// Two boundary modes:
+//   REFERENCE: interior-only updates, ghost-layer fgBoundary (lbm.cu:365-401)
+//              and zero-gradient phi (lbm.cu:403-425) with the edge race
+//              resolved in serial order z-faces, then y, then x.
+//   PERIODIC : every cell updated, every stencil and stream wraps (the form
+//              the benchmark cases use; no reference counterpart).
+// Extension with no reference counterpart: a phase relaxation time tau_g.
+//   g* = (1-omega_g)*g + omega_g*(geq + W*Ai); at omega_g == 1 the kernel
+//   takes the literal reference form (lbm.cu:324-326) without reading g.
+// Weight tables: LITERAL = the reference's float literals (var.cu:40-45),
+// whose float values sum to 1 + 2^-26; UNIT_SUM = the nearest float
+// weights whose exact sum is 1 (w0 one ulp above 1/3, w1 one ulp below 1/18,
+// w2 = w1/2), which removes a systematic +1.49e-8 per-step mass gain.
+//
+// Built with -ffp-contract=off so every product and sum is rounded exactly as
+// written; the OpenMP build only splits the cell loops, so it is bitwise equal
+// to the serial build.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace mcbo {
+
+enum BcMode { PERIODIC = 0, REFERENCE = 1 };
+enum WeightSet { W_LITERAL = 0, W_UNIT_SUM = 1 };
+
+// var.cu:29-31 (D3Q19 velocity set, same link order).
+static const int CIX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+static const int CIY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+static const int CIZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+
+struct Params {
+    int nx, ny, nz;
+    int bc_mode;        // BcMode
+    int weight_set;     // WeightSet
+    int moving_wall;    // 1: also run the dead movingWallBC (lbm.cu:430-461)
+    double tau_f;       // H_TAU (var.cu:21)
+    double tau_g;       // 1.0 reproduces the reference phase collision
+    double sigma;       // H_SIGMA (var.cu:25)
+    double sharp_c;     // H_SHARP_C = 0.15f*3.0f (var.cu:24); <0 selects it
+};
+
+template <class T> struct Consts {
+    T cssq, omega, sharp_c, sigma, omega_g;
+    bool tau_g_is_one;
+    T w[19];
+};
+
+// Float constants are evaluated in float exactly as var.cu:21-25 does on the
+// host; the FP64 oracle evaluates the same expressions in double.
+template <class T> Consts<T> make_consts(const Params& p) {
+    Consts<T> c;
+    c.cssq = T(1) / T(3);                                   // var.cu:22
+    c.omega = T(1) / T(p.tau_f);                            // var.cu:23
+    c.sharp_c = p.sharp_c < 0 ? T(0.15) * T(3) : T(p.sharp_c);  // var.cu:24
+    c.sigma = T(p.sigma);                                   // var.cu:25
+    c.tau_g_is_one = (p.tau_g == 1.0);
+    c.omega_g = T(1) / T(p.tau_g);
+    T w0 = T(1) / T(3), w1 = T(1) / T(18), w2 = T(1) / T(36);  // var.cu:40-45
+    if (p.weight_set == W_UNIT_SUM) {
+        if (sizeof(T) == 4) {
+            w0 = (T)std::ldexp(11184812.0, -25);
+            w1 = (T)std::ldexp(14913080.0, -28);
+            w2 = (T)std::ldexp(14913080.0, -29);
+        } else {
+            w1 = T(1) / T(18);
+            w2 = w1 / T(2);
+            w0 = T(1) - T(12) * w1;
+        }
+    }
+    c.w[0] = w0;
+    for (int l = 1; l <= 6; ++l) c.w[l] = w1;
+    for (int l = 7; l < 19; ++l) c.w[l] = w2;
+    return c;
+}
+
+struct Bubble { double cx, cy, cz, r; };
+
+template <class T> struct State {
+    Params p;
+    Consts<T> c;
+    int NX, NY, NZ;
+    size_t N;
+    // var.cu:13-18: the reference's device buffers, one array each.
+    std::vector<T> f, g, g_out;
+    std::vector<T> normx, normy, normz, indicator;
+    std::vector<T> curvature, ffx, ffy, ffz;
+    std::vector<T> ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, rho, phi;
+
+    explicit State(const Params& pp) : p(pp), c(make_consts<T>(pp)) {
+        NX = p.nx; NY = p.ny; NZ = p.nz;
+        N = (size_t)NX * NY * NZ;
+        // var.cu:57-101: allocation with zero fill (cudaMemset) of every
+        // buffer the reference clears; the others are overwritten by init.
+        for (auto* v : fields()) v->assign(N, T(0));
+        f.assign(N * 19, T(0));
+        g.assign(N * 19, T(0));
+        g_out.assign(N * 19, T(0));
+        build_tables();
+    }
+    std::vector<std::vector<T>*> fields() {
+        return {&normx, &normy, &normz, &indicator, &curvature, &ffx, &ffy, &ffz,
+                &ux, &uy, &uz, &pxx, &pyy, &pzz, &pxy, &pxz, &pyz, &rho, &phi};
+    }
+    // header.cuh:20-21 (64-bit so large lattices do not overflow).
+    size_t idx3(int i, int j, int k) const { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); }
+    size_t idx4(int i, int j, int k, int l) const { return idx3(i, j, k) + (size_t)l * N; }
+    bool periodic() const { return p.bc_mode == PERIODIC; }
+    bool interior(int i, int j, int k) const {
+        return !(i == 0 || i == NX - 1 || j == 0 || j == NY - 1 || k == 0 || k == NZ - 1);
+    }
+    // Cells a bulk kernel updates: all cells (PERIODIC) or the interior
+    // guard of lbm.cu:14 and its copies.
+    bool active(int i, int j, int k) const { return periodic() || interior(i, j, k); }
+    static int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+    // Neighbour coordinate: interior cells never leave the domain in
+    // REFERENCE mode, so the wrap only acts in PERIODIC mode.
+    int wx(int a) const { return wrap(a, NX); }
+    int wy(int a) const { return wrap(a, NY); }
+    int wz(int a) const { return wrap(a, NZ); }
+    // Per-axis tables of wrapped coordinates for offsets -1, 0, +1, and the
+    // 19 neighbour indices x + c_l of a cell (same values as wx/wy/wz).
+    std::vector<int> tx[3], ty[3], tz[3];
+    void build_tables() {
+        for (int d = 0; d < 3; ++d) {
+            tx[d].resize(NX); ty[d].resize(NY); tz[d].resize(NZ);
+            for (int a = 0; a < NX; ++a) tx[d][a] = wrap(a + d - 1, NX);
+            for (int a = 0; a < NY; ++a) ty[d][a] = wrap(a + d - 1, NY);
+            for (int a = 0; a < NZ; ++a) tz[d][a] = wrap(a + d - 1, NZ);
+        }
+    }
+    void neighbours(int i, int j, int k, size_t out[19]) const {
+        #pragma GCC unroll 19
+        for (int l = 0; l < 19; ++l)
+            out[l] = idx3(tx[CIX[l] + 1][i], ty[CIY[l] + 1][j], tz[CIZ[l] + 1][k]);
+    }
+};
+
+// ---------------------------------------------------------------- init.cu
+
+// initTensor (init.cu:3-25): rho = 1 and the six Pi components = 1.
+template <class T> void init_tensor(State<T>& s) {
+    for (size_t n = 0; n < s.N; ++n) {
+        T val = T(1);
+        s.pxx[n] = val; s.pyy[n] = val; s.pzz[n] = val;
+        s.pxy[n] = val; s.pxz[n] = val; s.pyz[n] = val;
+        s.rho[n] = val;
+    }
+}
+
+// One tanh bubble profile (init.cu:39-46), generalised to a centre, radius
+// and width. dx = i - cx matches `i - NX*0.5f` for the reference centre.
+// PERIODIC mode measures minimum-image distances.
+template <class T> T bubble_phi(const State<T>& s, int i, int j, int k, const Bubble& b, T width) {
+    T dx = T(i) - T(b.cx);
+    T dy = T(j) - T(b.cy);
+    T dz = T(k) - T(b.cz);
+    if (s.periodic()) {
+        T hx = T(s.NX) * T(0.5), hy = T(s.NY) * T(0.5), hz = T(s.NZ) * T(0.5);
+        if (dx > hx) dx -= T(s.NX); else if (dx < -hx) dx += T(s.NX);
+        if (dy > hy) dy -= T(s.NY); else if (dy < -hy) dy += T(s.NY);
+        if (dz > hz) dz -= T(s.NZ); else if (dz < -hz) dz += T(s.NZ);
+    }
+    T Ri = std::sqrt(dx * dx + dy * dy + dz * dz);
+    T bubble_radius = T(b.r);
+    return T(0.5) + T(0.5) * std::tanh(T(2) * (bubble_radius - Ri) / width);
+}
+
+// initPhase (init.cu:27-49): interior only in REFERENCE mode (faces keep the
+// memset zero, var.cu:86); several bubbles combine by maximum.
+template <class T> void init_phase(State<T>& s, const std::vector<Bubble>& bubbles, T width) {
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                T v = T(0);
+                for (size_t b = 0; b < bubbles.size(); ++b) {
+                    T pb = bubble_phi(s, i, j, k, bubbles[b], width);
+                    v = (b == 0) ? pb : std::max(v, pb);
+                }
+                s.phi[s.idx3(i, j, k)] = v;
+            }
+}
+
+// initDist (init.cu:51-80): f_l = W_l*rho, g_l = W_l*phi on every cell.
+template <class T> void init_dist(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    for (size_t n = 0; n < s.N; ++n) {
+        T rho_val = s.rho[n], phi_val = s.phi[n];
+        #pragma GCC unroll 19
+        for (int l = 0; l < 19; ++l) s.f[n + l * s.N] = c.w[l] * rho_val;
+        #pragma GCC unroll 19
+        for (int l = 0; l < 19; ++l) s.g[n + l * s.N] = c.w[l] * phi_val;
+    }
+}
+
+// ----------------------------------------------------------------- lbm.cu
+
+// phiCalc (lbm.cu:5-26).
+template <class T> void phi_calc(State<T>& s) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                T sum = T(0);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) sum += s.g[s.idx4(i, j, k, l)];
+                s.phi[s.idx3(i, j, k)] = sum;
+            }
+}
+
+template <class T> inline T fmax_ref(T a, T b) { return std::fmax(a, b); }
+
+// gradCalc (lbm.cu:34-72). rsqrtf -> 1/sqrt (see header).
+template <class T> void grad_calc(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    const T eps = T(1e-9f);  // fmaxf(.., 1e-9) converts the literal to float
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                size_t nb[19];
+                s.neighbours(i, j, k, nb);
+                T gx = T(0), gy = T(0), gz = T(0);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    T val = s.phi[nb[l]];
+                    T coef = T(3) * c.w[l];
+                    gx += coef * T(CIX[l]) * val;
+                    gy += coef * T(CIY[l]) * val;
+                    gz += coef * T(CIZ[l]) * val;
+                }
+                T gmag_sq = gx * gx + gy * gy + gz * gz;
+                T factor = T(1) / std::sqrt(fmax_ref(gmag_sq, eps));
+                s.normx[idx] = gx * factor;
+                s.normy[idx] = gy * factor;
+                s.normz[idx] = gz * factor;
+                s.indicator[idx] = gmag_sq * factor;
+            }
+}
+
+// curvatureCalc (lbm.cu:80-125).
+template <class T> void curvature_calc(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                T nx_ = s.normx[idx], ny_ = s.normy[idx], nz_ = s.normz[idx];
+                T ind_ = s.indicator[idx];
+                T curv = T(0);
+                size_t nb[19];
+                s.neighbours(i, j, k, nb);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    size_t o = nb[l];
+                    T coef = T(3) * c.w[l];
+                    curv -= coef * (T(CIX[l]) * s.normx[o] + T(CIY[l]) * s.normy[o] + T(CIZ[l]) * s.normz[o]);
+                }
+                T mult = c.sigma * curv;
+                s.curvature[idx] = curv;
+                s.ffx[idx] = mult * nx_ * ind_;
+                s.ffy[idx] = mult * ny_ * ind_;
+                s.ffz[idx] = mult * nz_ * ind_;
+            }
+}
+
+// momentiCalc (lbm.cu:133-225).
+template <class T> void momenti_calc(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    const T half = T(0.5);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                T fVal[19], fneq[19];
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) fVal[l] = s.f[s.idx4(i, j, k, l)];
+                T rhoVal = T(0);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) rhoVal += fVal[l];
+                T invRho = T(1) / rhoVal;
+                T sumUx = invRho * (fVal[1] - fVal[2] + fVal[7] - fVal[8] + fVal[9] - fVal[10] + fVal[13] - fVal[14] + fVal[15] - fVal[16]);
+                T sumUy = invRho * (fVal[3] - fVal[4] + fVal[7] - fVal[8] + fVal[11] - fVal[12] + fVal[14] - fVal[13] + fVal[17] - fVal[18]);
+                T sumUz = invRho * (fVal[5] - fVal[6] + fVal[9] - fVal[10] + fVal[11] - fVal[12] + fVal[16] - fVal[15] + fVal[18] - fVal[17]);
+                T ffx_val = s.ffx[idx], ffy_val = s.ffy[idx], ffz_val = s.ffz[idx];
+                T halfFx = ffx_val * half * invRho;
+                T halfFy = ffy_val * half * invRho;
+                T halfFz = ffz_val * half * invRho;
+                T uxVal = sumUx + halfFx, uyVal = sumUy + halfFy, uzVal = sumUz + halfFz;
+                T invCssq = T(1) / c.cssq;
+                T uu = half * (uxVal * uxVal + uyVal * uyVal + uzVal * uzVal) * invCssq;
+                T invRhoCssq = T(1) / (rhoVal * c.cssq);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    T udotc = (uxVal * T(CIX[l]) + uyVal * T(CIY[l]) + uzVal * T(CIZ[l])) * invCssq;
+                    T eqBase = rhoVal * (udotc + half * udotc * udotc - uu);
+                    T common = c.w[l] * (rhoVal + eqBase);
+                    T HeF = half * common * ((T(CIX[l]) - uxVal) * ffx_val +
+                                             (T(CIY[l]) - uyVal) * ffy_val +
+                                             (T(CIZ[l]) - uzVal) * ffz_val) * invRhoCssq;
+                    T feq = common - HeF;
+                    fneq[l] = fVal[l] - feq;
+                }
+                s.pxx[idx] = fneq[1] + fneq[2] + fneq[7] + fneq[8] + fneq[9] + fneq[10] + fneq[13] + fneq[14] + fneq[15] + fneq[16];
+                s.pyy[idx] = fneq[3] + fneq[4] + fneq[7] + fneq[8] + fneq[11] + fneq[12] + fneq[13] + fneq[14] + fneq[17] + fneq[18];
+                s.pzz[idx] = fneq[5] + fneq[6] + fneq[9] + fneq[10] + fneq[11] + fneq[12] + fneq[15] + fneq[16] + fneq[17] + fneq[18];
+                s.pxy[idx] = fneq[7] - fneq[13] + fneq[8] - fneq[14];
+                s.pxz[idx] = fneq[9] - fneq[15] + fneq[10] - fneq[16];
+                s.pyz[idx] = fneq[11] - fneq[17] + fneq[12] - fneq[18];
+                s.ux[idx] = uxVal; s.uy[idx] = uyVal; s.uz[idx] = uzVal;
+                s.rho[idx] = rhoVal;
+            }
+}
+
+// collisionFluid (lbm.cu:233-288): regularized BGK + He forcing, pushed to
+// x + c_l. It never reads f, so the in-place push is race-free.
+template <class T> void collision_fluid(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    const T half = T(0.5);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                T ux_val = s.ux[idx], uy_val = s.uy[idx], uz_val = s.uz[idx], rho_val = s.rho[idx];
+                T ffx_val = s.ffx[idx], ffy_val = s.ffy[idx], ffz_val = s.ffz[idx];
+                T pxx_val = s.pxx[idx], pyy_val = s.pyy[idx], pzz_val = s.pzz[idx];
+                T pxy_val = s.pxy[idx], pxz_val = s.pxz[idx], pyz_val = s.pyz[idx];
+                const T CSSQ = c.cssq;
+                T uu = half * (ux_val * ux_val + uy_val * uy_val + uz_val * uz_val) / CSSQ;
+                T invRhoCssq = T(1) / (rho_val * CSSQ);
+                T invCssq = T(1) / CSSQ;
+                size_t nb[19];
+                s.neighbours(i, j, k, nb);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    T udotc = (ux_val * T(CIX[l]) + uy_val * T(CIY[l]) + uz_val * T(CIZ[l])) * invCssq;
+                    T feq = (c.w[l] * (rho_val + rho_val * (udotc + half * udotc * udotc - uu)));
+                    T HeF = half * feq * ((T(CIX[l]) - ux_val) * ffx_val +
+                                          (T(CIY[l]) - uy_val) * ffy_val +
+                                          (T(CIZ[l]) - uz_val) * ffz_val) * invRhoCssq;
+                    T fneq = (c.w[l] / (T(2) * CSSQ * CSSQ)) *
+                           ((T(CIX[l] * CIX[l]) - CSSQ) * pxx_val +
+                            (T(CIY[l] * CIY[l]) - CSSQ) * pyy_val +
+                            (T(CIZ[l] * CIZ[l]) - CSSQ) * pzz_val +
+                            T(2) * T(CIX[l]) * T(CIY[l]) * pxy_val +
+                            T(2) * T(CIX[l]) * T(CIZ[l]) * pxz_val +
+                            T(2) * T(CIY[l]) * T(CIZ[l]) * pyz_val);
+                    s.f[nb[l] + (size_t)l * s.N] = feq + (T(1) - c.omega) * fneq + HeF;
+                }
+            }
+}
+
+// collisionPhase (lbm.cu:290-329), in place. tau_g extension: see header.
+template <class T> void collision_phase(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (!s.active(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                T ux_val = s.ux[idx], uy_val = s.uy[idx], uz_val = s.uz[idx];
+                T phi_val = s.phi[idx];
+                T normx_val = s.normx[idx], normy_val = s.normy[idx], normz_val = s.normz[idx];
+                T invCSSQ = T(1) / c.cssq;
+                T phi_norm = c.sharp_c * phi_val * (T(1) - phi_val);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    size_t i4 = s.idx4(i, j, k, l);
+                    T udotc = (ux_val * T(CIX[l]) + uy_val * T(CIY[l]) + uz_val * T(CIZ[l])) * invCSSQ;
+                    T geq = c.w[l] * phi_val * (T(1) + udotc);
+                    T Ai = phi_norm * (T(CIX[l]) * normx_val + T(CIY[l]) * normy_val + T(CIZ[l]) * normz_val);
+                    T gpost = geq + c.w[l] * Ai;
+                    if (c.tau_g_is_one)
+                        s.g[i4] = gpost;
+                    else
+                        s.g[i4] = (T(1) - c.omega_g) * s.g[i4] + c.omega_g * gpost;
+                }
+            }
+}
+
+// streamingCalc (lbm.cu:337-357) then swap(d_g, d_g_out) (main.cu:156).
+template <class T> void streaming_calc(State<T>& s) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i)
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    int si = s.tx[1 - CIX[l]][i];
+                    int sj = s.ty[1 - CIY[l]][j];
+                    int sk = s.tz[1 - CIZ[l]][k];
+                    s.g_out[s.idx4(i, j, k, l)] = s.g[s.idx4(si, sj, sk, l)];
+                }
+    std::swap(s.g, s.g_out);
+}
+
+// fgBoundary (lbm.cu:365-401), REFERENCE mode only.
+template <class T> void fg_boundary(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    for (int k = 0; k < s.NZ; ++k)
+        for (int j = 0; j < s.NY; ++j)
+            for (int i = 0; i < s.NX; ++i) {
+                if (s.interior(i, j, k)) continue;
+                size_t idx = s.idx3(i, j, k);
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    int ni = i + CIX[l], nj = j + CIY[l], nk = k + CIZ[l];
+                    if (ni >= 0 && ni < s.NX && nj >= 0 && nj < s.NY && nk >= 0 && nk < s.NZ)
+                        s.f[s.idx4(ni, nj, nk, l)] = s.rho[idx] * c.w[l];
+                }
+                #pragma GCC unroll 19
+                for (int l = 0; l < 19; ++l) {
+                    int ni = i + CIX[l], nj = j + CIY[l], nk = k + CIZ[l];
+                    if (ni >= 0 && ni < s.NX && nj >= 0 && nj < s.NY && nk >= 0 && nk < s.NZ)
+                        s.g[s.idx4(ni, nj, nk, l)] = s.phi[idx] * c.w[l];
+                }
+            }
+}
+
+// boundaryConditions (lbm.cu:403-425), REFERENCE mode only. The reference
+// runs the three face pairs concurrently (edge/corner race); here they run
+// in the fixed order z, y, x, each reading the previous result.
+template <class T> void boundary_conditions(State<T>& s) {
+    auto& phi = s.phi;
+    int NX = s.NX, NY = s.NY, NZ = s.NZ;
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
+            phi[s.idx3(i, j, 0)] = phi[s.idx3(i, j, 1)];
+            phi[s.idx3(i, j, NZ - 1)] = phi[s.idx3(i, j, NZ - 2)];
+        }
+    for (int k = 0; k < NZ; ++k)
+        for (int i = 0; i < NX; ++i) {
+            phi[s.idx3(i, 0, k)] = phi[s.idx3(i, 1, k)];
+            phi[s.idx3(i, NY - 1, k)] = phi[s.idx3(i, NY - 2, k)];
+        }
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j) {
+            phi[s.idx3(0, j, k)] = phi[s.idx3(1, j, k)];
+            phi[s.idx3(NX - 1, j, k)] = phi[s.idx3(NX - 2, j, k)];
+        }
+}
+
+// movingWallBC (lbm.cu:430-461). Dead in the reference (nothing reads f on
+// the j faces); kept so a test can show it changes no observable.
+template <class T> void moving_wall_bc(State<T>& s) {
+    const Consts<T> c = s.c;  // local copy: stores to fields cannot alias it
+    const T U_wall = T(0.03);
+    // Its 2-D launch (main.cu:145) maps k to blockIdx.y*8+threadIdx.y, so it
+    // only reaches k < 8*ceil(NY/8).
+    int kmax = std::min(s.NZ, ((s.NY + 7) / 8) * 8);
+    for (int k = 0; k < kmax; ++k)
+        for (int i = 0; i < s.NX; ++i) {
+            int jb = 0, jt = s.NY - 1;
+            T rb = s.rho[s.idx3(i, jb, k)];
+            #pragma GCC unroll 19
+            for (int l = 0; l < 19; ++l) {
+                T cu = (T(CIX[l]) * (-U_wall)) / c.cssq;
+                s.f[s.idx4(i, jb, k, l)] = rb * c.w[l] * (T(1) + cu);
+            }
+            T rt = s.rho[s.idx3(i, jt, k)];
+            #pragma GCC unroll 19
+            for (int l = 0; l < 19; ++l) {
+                T cu = (T(CIX[l]) * U_wall) / c.cssq;
+                s.f[s.idx4(i, jt, k, l)] = rt * c.w[l] * (T(1) + cu);
+            }
+        }
+}
+
+// Phases a6-a9 of one iteration: what the reference computes at the start
+// of each step (main.cu:79-119). After N steps these give the observables.
+template <class T> void macros(State<T>& s) {
+    phi_calc(s);
+    grad_calc(s);
+    curvature_calc(s);
+    momenti_calc(s);
+}
+
+// One full iteration of the reference time loop (main.cu:74-182), with the
+// collisions serialised after momentiCalc (the reference's stream race R1).
+template <class T> void step(State<T>& s) {
+    macros(s);
+    collision_fluid(s);
+    collision_phase(s);
+    if (s.p.bc_mode == REFERENCE && s.p.moving_wall) moving_wall_bc(s);
+    streaming_calc(s);
+    if (s.p.bc_mode == REFERENCE) {
+        fg_boundary(s);
+        boundary_conditions(s);
+    }
+}
+
+}  // namespace mcbo

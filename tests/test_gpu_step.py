@@ -1,0 +1,145 @@
+"""GPU parity of the full time step against the oracle (FP32 transcription).
+
+Both the fused path (k_phase + k_collide) and the staged path (the
+reference's kernel sequence, mcb_ops.h) must reproduce the transcription
+bit for bit: the kernels round every operation as the transcription does
+(-ffp-contract=off, IEEE division/sqrt, denormals kept). The stated
+tolerance of the path (1e-5 relative L2 on phi, rho, u after 1000 steps)
+is asserted too, so a lost bit-exactness shows up as a named failure with
+its size, not as a silent pass.
+"""
+import numpy as np
+import pytest
+
+from conftest import f32_bubbles, oracle_for, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+OBS = ("phi", "rho", "ux", "uy", "uz")
+
+
+def run_pair(mcb, params, bubbles, steps, width=3.0):
+    lat = mcb.Lattice(params)
+    lat.init_bubbles(bubbles, width)
+    o = oracle_for(params)
+    o.init(f32_bubbles(bubbles), width)
+    lat.step(steps)
+    o.step(steps)
+    got = lat.observables()
+    o.macros()
+    want = {k: o.field(k).copy() for k in OBS}
+    return lat, o, got, want
+
+
+def assert_parity(got, want, label, exact=True):
+    for k in OBS:
+        err = rel_l2(got[k], want[k])
+        assert err <= TOL, f"{label}: {k} rel L2 {err:.3e} > {TOL}"
+        if exact:
+            nbad = int(np.count_nonzero(got[k] != want[k]))
+            assert nbad == 0, f"{label}: {k} differs from the oracle in {nbad} cells (rel L2 {err:.3e})"
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+@pytest.mark.parametrize("shape,tau_g", [((32, 32, 32), 1.0), ((48, 40, 36), 0.8), ((20, 24, 17), 1.0)])
+def test_periodic_short_bitexact(gpu_lib, path, shape, tau_g):
+    mcb = gpu_lib
+    nx, ny, nz = shape
+    p = mcb.make_params(nx, ny, nz, tau_f=0.55, tau_g=tau_g, sigma=0.1,
+                        path=mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED)
+    bubbles = [(nx * 0.5 + 0.3, ny * 0.5 - 1.2, nz * 0.5 + 0.7, min(shape) * 0.25)]
+    lat, o, got, want = run_pair(mcb, p, bubbles, 20)
+    assert_parity(got, want, f"{path} {shape} tau_g={tau_g}")
+    lat.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_cfg0_1000_steps(gpu_lib, path):
+    """64^3, R=10 centred, tau_f 0.55, tau_g 0.8, sigma 0.1, 1000 steps."""
+    mcb = gpu_lib
+    c = mcb.case_preset("cfg0")
+    c.params.path = mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED
+    lat, o, got, want = run_pair(mcb, c.params, mcb.case_bubbles(c), 1000)
+    assert_parity(got, want, f"cfg0 {path}")
+    # Phase conservation: sum(phi) after 1000 steps vs the initial bubble.
+    phi0 = mcb.host_init_phi(c.params, mcb.case_bubbles(c), c.width).astype(np.float64).sum()
+    s, ke, finite = lat.reduce()
+    assert finite
+    assert abs(s - phi0) / phi0 <= 1e-7, f"sum(phi) drift {abs(s - phi0) / phi0:.3e}"
+    assert abs(ke - o.kinetic()) <= 1e-9 * max(1.0, abs(ke))
+    lat.close()
+
+
+def test_fused_tilings_agree(gpu_lib):
+    """Every tile shape and z-chunk gives the same bits."""
+    mcb = gpu_lib
+    p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=0.8, sigma=0.05, path=mcb.PATH_FUSED)
+    bubbles = [(20.5, 17.0, 14.2, 7.5)]
+    ref = None
+    for tx, ty in mcb.tilings():
+        for zc in (1, 5, 28):
+            lat = mcb.Lattice(p)
+            lat.init_bubbles(bubbles)
+            lat.set_tiling(tx, ty, zc)
+            lat.step(7)
+            obs = lat.observables()
+            lat.close()
+            if ref is None:
+                ref = obs
+            for k in OBS:
+                assert np.array_equal(obs[k], ref[k]), f"tiling {tx}x{ty} zc={zc}: {k} differs"
+
+
+def test_populations_roundtrip_and_layout(gpu_lib):
+    """F/G readout is the reference layout (post-stream, pre-collision) for both paths."""
+    mcb = gpu_lib
+    shape = (24, 20, 16)
+    bubbles = [(12.0, 10.0, 8.0, 5.0)]
+    outs = {}
+    for path in ("fused", "staged"):
+        p = mcb.make_params(*shape, tau_f=0.55, tau_g=0.8, path=mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED)
+        lat = mcb.Lattice(p)
+        lat.init_bubbles(bubbles)
+        lat.step(5)
+        outs[path] = (lat.read("f"), lat.read("g"))
+        # write back what was read: the next step must be unchanged
+        lat.write("f", outs[path][0])
+        lat.write("g", outs[path][1])
+        lat.step(1)
+        outs[path + "+1"] = lat.observables()
+        lat.close()
+    assert np.array_equal(outs["fused"][0], outs["staged"][0])
+    assert np.array_equal(outs["fused"][1], outs["staged"][1])
+    o = oracle_for(mcb.make_params(*shape, tau_f=0.55, tau_g=0.8))
+    o.init(f32_bubbles(bubbles))
+    o.step(5)
+    assert np.array_equal(outs["fused"][0].reshape(19, -1), o.field("f").reshape(19, -1))
+    assert np.array_equal(outs["fused"][1].reshape(19, -1), o.field("g").reshape(19, -1))
+    o.step(1)
+    o.macros()
+    for k in OBS:
+        assert np.array_equal(outs["fused+1"][k], o.field(k))
+        assert np.array_equal(outs["staged+1"][k], o.field(k))
+
+
+def test_reference_mode_staged(gpu_lib):
+    """Ghost-layer boundaries of the reference (staged path) against the oracle."""
+    mcb = gpu_lib
+    p = mcb.make_params(24, 20, 40, bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL, tau_f=0.505,
+                        tau_g=1.0, sigma=0.1)
+    bubbles = [(12.0, 10.0, 20.0, 6.0)]
+    lat = mcb.Lattice(p)
+    lat.init_bubbles(bubbles)
+    o = oracle_for(p)
+    o.init(f32_bubbles(bubbles))
+    for _ in range(3):
+        lat.step(10)
+        o.step(10)
+        # d_phi after an iteration, with boundaryConditions applied (the saved file)
+        assert np.array_equal(lat.read("phi"), o.field("phi"))
+    got = lat.observables()
+    o.macros()
+    for k in OBS:
+        assert np.array_equal(got[k], o.field(k)), k
+    lat.close()
