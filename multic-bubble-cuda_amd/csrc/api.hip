@@ -34,6 +34,7 @@ struct mcb_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> tev;   // per-kernel timing events
     Tiling tiling{64, 4, 16};
     std::string err;
     std::vector<void*> allocs;
@@ -78,10 +79,16 @@ mcb_status hip_fail(mcb_ctx* c, hipError_t e, const char* where) {
 // plane starts 1 KiB-aligned.
 long long plane_stride(long long n) { return (n + 255) / 256 * 256; }
 
+// The fused kernels address all 19 planes of a buffer through one 32-bit
+// buffer resource (lattices up to ~56.5M cells).
+bool fused_fits(const mcb_params& p) {
+    return plane_stride((long long)p.nx * p.ny * p.nz) <= kMaxFusedPlane;
+}
+
 int resolve_path(const mcb_params& p) {
     if (p.path == MCB_PATH_STAGED) return MCB_PATH_STAGED;
     if (p.path == MCB_PATH_FUSED) return MCB_PATH_FUSED;
-    return p.bc_mode == MCB_BC_PERIODIC ? MCB_PATH_FUSED : MCB_PATH_STAGED;
+    return p.bc_mode == MCB_BC_PERIODIC && fused_fits(p) ? MCB_PATH_FUSED : MCB_PATH_STAGED;
 }
 
 size_t footprint(const mcb_params& p, int path) {
@@ -234,7 +241,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
     mcb_status st = host_validate(p, &why);
     if (st != MCB_OK) return st;
     const int path = resolve_path(*p);
-    if (path == MCB_PATH_FUSED && p->bc_mode != MCB_BC_PERIODIC) return MCB_ERR_UNSUPPORTED;
+    if (path == MCB_PATH_FUSED && (p->bc_mode != MCB_BC_PERIODIC || !fused_fits(*p))) return MCB_ERR_UNSUPPORTED;
     if (p->storage != MCB_STORE_FP32) return MCB_ERR_UNSUPPORTED;
 
     int count = 0;
@@ -276,7 +283,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
         for (auto* q : scal)
             if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
         // Default tiling: widest x tile the lattice fills.
-        c->tiling = p->nx >= 64 ? Tiling{64, 4, 16} : (p->nx >= 32 ? Tiling{32, 8, 16} : Tiling{16, 16, 16});
+        c->tiling = p->nx >= 64 ? Tiling{64, 8, 16} : (p->nx >= 32 ? Tiling{32, 8, 16} : Tiling{16, 16, 16});
     } else {
         // The reference layout: planes of exactly n cells (src/header.cuh:21).
         c->geo.plane = n;
@@ -305,6 +312,7 @@ mcb_status mcb_destroy(mcb_ctx* c) {
     for (void* q : c->allocs) (void)hipFree(q);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->tev) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MCB_OK;
@@ -469,13 +477,30 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
     if (st != MCB_OK) return st;
     if (c->path != MCB_PATH_FUSED) return fail(c, MCB_ERR_UNSUPPORTED, "kernel times need the fused path");
     if (!out_ms || n_out < 2 || nsteps < 1) return fail(c, MCB_ERR_INVALID_ARG, "bad arguments");
+    // Three events per step, recorded back to back on the stream and read
+    // once at the end, so the timed steps run without host round trips.
+    const size_t need = 3 * (size_t)nsteps;
+    while (c->tev.size() < need) {
+        hipEvent_t e = nullptr;
+        MCB_HIP(c, hipEventCreate(&e));
+        c->tev.push_back(e);
+    }
+    for (int t = 0; t < nsteps; ++t) {
+        hipEvent_t* e = &c->tev[3 * (size_t)t];
+        MCB_HIP(c, hipEventRecord(e[0], c->stream));
+        MCB_HIP(c, launch_phase(c->g[c->cur], c->phi, c->geo, c->stream));
+        MCB_HIP(c, hipEventRecord(e[1], c->stream));
+        MCB_HIP(c, launch_collide(collide_args(c), step_mode(c), c->tiling, c->stream));
+        MCB_HIP(c, hipEventRecord(e[2], c->stream));
+        c->cur ^= 1;
+    }
+    MCB_HIP(c, hipEventSynchronize(c->tev[need - 1]));
     double acc[2] = {0.0, 0.0};
     for (int t = 0; t < nsteps; ++t) {
-        if ((st = fused_step(c, true)) != MCB_OK) return st;
-        MCB_HIP(c, hipEventSynchronize(c->ev[2]));
+        hipEvent_t* e = &c->tev[3 * (size_t)t];
         float a = 0.0f, b = 0.0f;
-        MCB_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-        MCB_HIP(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        MCB_HIP(c, hipEventElapsedTime(&a, e[0], e[1]));
+        MCB_HIP(c, hipEventElapsedTime(&b, e[1], e[2]));
         acc[0] += a;
         acc[1] += b;
     }

@@ -29,6 +29,52 @@ template <int C> __device__ __forceinline__ int shift(int a, int n) {
     else return a;
 }
 
+// Element access at a 32-bit element offset from a wave-uniform base. The
+// byte offset is formed in 32 bits, so the access compiles to a global
+// load/store with the base in scalar registers and one 32-bit vector offset
+// (no per-lane 64-bit address arithmetic). Needs elem * 4 < 2^32, i.e. planes
+// of fewer than 2^30 cells (enforced by parameter validation).
+__device__ __forceinline__ float ld32(const float* __restrict__ base, unsigned elem) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (elem << 2));
+}
+__device__ __forceinline__ void st32(float* __restrict__ base, unsigned elem, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (elem << 2)) = v;
+}
+
+// The 19 population planes of one buffer behind a single buffer resource:
+// base address in four scalar registers, plane l selected by the scalar
+// offset l * plane_bytes, the cell by a 32-bit vector byte offset. This keeps
+// one descriptor live instead of 19 64-bit plane pointers. Accesses beyond
+// the resource's extent read 0 and drop stores instead of faulting. Needs
+// 19 * plane * 4 < 2^32 (checked on the host before the fused path is used).
+constexpr int kBufferConfigWord = 0x00020000;   // raw 32-bit buffer, gfx9 family
+// Cache-policy bits of the population stores (gfx950: bit 1 = non-temporal).
+// The post-collision planes are read back one full pass later, long after a
+// 5 GB working set has cycled through L2 and the 256 MB Infinity Cache, so
+// they are written non-temporally: measured 0.31 -> 0.28 ms for the following
+// phase pass at 256^3, because fewer dirty lines are evicted during it.
+#ifndef MCB_STORE_AUX
+#define MCB_STORE_AUX 2
+#endif
+struct Planes {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned plane_bytes;
+    __device__ __forceinline__ Planes(const float* base, unsigned plane_elems)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)(19u * plane_elems * 4u),
+                                              kBufferConfigWord)),
+          plane_bytes(plane_elems * 4u) {}
+    template <int L> __device__ __forceinline__ float ld(unsigned elem) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, elem << 2, L * plane_bytes, 0));
+    }
+    template <int L> __device__ __forceinline__ void st(unsigned elem, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, elem << 2, L * plane_bytes,
+                                              MCB_STORE_AUX);
+    }
+};
+
+// Largest plane stride (cells) the fused kernels address through Planes.
+constexpr long long kMaxFusedPlane = 0xFFFFFFFFLL / (19 * 4);
+
 inline mcb_status hip_status(hipError_t e) {
     if (e == hipSuccess) return MCB_OK;
     if (e == hipErrorOutOfMemory) return MCB_ERR_OUT_OF_MEMORY;

@@ -25,26 +25,46 @@ template <int C, class T> __device__ __forceinline__ T pick(T m, T z, T p) {
     else return z;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), and each XCD has its own L2. xcd_order renumbers them so every
+// XCD walks one contiguous range of tiles: x-neighbouring tiles, which share
+// the cache lines that the x-shifted gathers only partly use, then meet in one
+// L2 instead of fetching those lines from HBM twice. Bijective on [0, total).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_order(unsigned lid, unsigned total) {
+    const unsigned q = total / kXcds, r = total % kXcds;
+    const unsigned xcd = lid % kXcds, slot = lid / kXcds;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
+
 // ----------------------------------------------------------------- k_phase
 
 constexpr int PBX = 64, PBY = 4;
 
-__global__ __launch_bounds__(PBX * PBY) void k_phase(const float* __restrict__ g, float* __restrict__ phi, Geo G) {
-    const int x = blockIdx.x * PBX + threadIdx.x;
-    const int y = blockIdx.y * PBY + threadIdx.y;
-    const int z = blockIdx.z;
+// 1-D grid of gx*gy*nz workgroups of 64x4 cells, renumbered by xcd_order.
+__global__ __launch_bounds__(PBX * PBY) void k_phase(const float* __restrict__ g, float* __restrict__ phi, Geo G,
+                                                     unsigned gx, unsigned gy) {
+    unsigned t = xcd_order(blockIdx.x, gridDim.x);
+    const unsigned bx = t % gx;
+    t /= gx;
+    const unsigned by = t % gy;
+    const int z = (int)(t / gy);
+    const int x = (int)(bx * PBX + threadIdx.x);
+    const int y = (int)(by * PBY + threadIdx.y);
     if (x >= G.nx || y >= G.ny) return;
     // Offsets of x-1, x, x+1 / rows y-1, y, y+1 / planes z-1, z, z+1.
-    const int xs[3] = {shift<-1>(x, G.nx), x, shift<1>(x, G.nx)};
-    const int ys[3] = {shift<-1>(y, G.ny) * G.nx, y * G.nx, shift<1>(y, G.ny) * G.nx};
-    const int zs[3] = {shift<-1>(z, G.nz) * G.nxy, z * G.nxy, shift<1>(z, G.nz) * G.nxy};
+    const unsigned xs[3] = {(unsigned)shift<-1>(x, G.nx), (unsigned)x, (unsigned)shift<1>(x, G.nx)};
+    const unsigned ys[3] = {(unsigned)(shift<-1>(y, G.ny) * G.nx), (unsigned)(y * G.nx),
+                            (unsigned)(shift<1>(y, G.ny) * G.nx)};
+    const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                            (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+    const Planes gp(g, (unsigned)G.plane);
     float sum = 0.0f;
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        const float* __restrict__ gl = g + (size_t)l * G.plane;
-        sum += gl[zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]];
+        sum += gp.ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
     });
-    phi[zs[1] + ys[1] + xs[1]] = sum;
+    st32(phi, zs[1] + ys[1] + xs[1], sum);
 }
 
 // --------------------------------------------------------------- k_collide
@@ -62,20 +82,27 @@ __global__ __launch_bounds__(TX * TY) void k_collide(CollideArgs a) {
 
     const Geo& G = a.geo;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int zb = blockIdx.z * a.zc;
+    // 1-D grid of gx*gy*gz tiles, x fastest, renumbered by xcd_order.
+    const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
+    unsigned t = xcd_order(blockIdx.x, gridDim.x);
+    const int x0 = (int)(t % gx) * TX;
+    t /= gx;
+    const int y0 = (int)(t % gy) * TY;
+    const int zb = (int)(t / gy) * a.zc;
     const int ze = min(zb + a.zc, G.nz);
     const Derived d(a.phys);
+    const Planes fsrc(a.fsrc, (unsigned)G.plane), fdst(a.fdst, (unsigned)G.plane);
+    const Planes gsrc(a.gsrc, (unsigned)G.plane), gdst(a.gdst, (unsigned)G.plane);
 
     // In-plane offsets of the halo phi elements this thread stages.
-    int poff[KP];
+    unsigned poff[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) {
         const int e = tid + q * NT;
-        int off = 0;
+        unsigned off = 0;
         if (e < PN) {
             const int lx = e % PX, ly = e / PX;
-            off = wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx);
+            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx));
         }
         poff[q] = off;
     }
@@ -86,7 +113,7 @@ __global__ __launch_bounds__(TX * TY) void k_collide(CollideArgs a) {
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
             const int e = tid + q * NT;
-            if (q < KP - 1 || e < PN) dst[e] = src[poff[q]];
+            if (q < KP - 1 || e < PN) dst[e] = ld32(src, poff[q]);
         }
     };
     auto nslot = [](int z) { return (z + 3) % 3; };  // z >= -3
@@ -119,8 +146,12 @@ __global__ __launch_bounds__(TX * TY) void k_collide(CollideArgs a) {
 
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
-    const int xs[3] = {valid ? shift<-1>(x, G.nx) : 0, x, valid ? shift<1>(x, G.nx) : 0};
-    const int ys[3] = {valid ? shift<-1>(y, G.ny) * G.nx : 0, y * G.nx, valid ? shift<1>(y, G.ny) * G.nx : 0};
+    // Unsigned 32-bit offsets: the gathers then use a scalar base + 32-bit
+    // vector offset instead of 64-bit per-lane addresses.
+    const unsigned xs[3] = {valid ? (unsigned)shift<-1>(x, G.nx) : 0u, (unsigned)x,
+                            valid ? (unsigned)shift<1>(x, G.nx) : 0u};
+    const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
+                            valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
 
     // Prologue: phi planes zb-2..zb+1, normals of planes zb-1 and zb.
     load_phi(zb - 2);
@@ -142,8 +173,9 @@ __global__ __launch_bounds__(TX * TY) void k_collide(CollideArgs a) {
         __syncthreads();
         if (!valid) continue;
 
-        const int zs[3] = {shift<-1>(z, G.nz) * G.nxy, z * G.nxy, shift<1>(z, G.nz) * G.nxy};
-        const int me = zs[1] + ys[1] + xs[1];
+        const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                                (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+        const unsigned me = zs[1] + ys[1] + xs[1];
         const int sm = nslot(z - 1), s0 = nslot(z), sp = nslot(z + 1);
         const int e0 = (ty + 1) * HX + (tx + 1);
         const float ncx = s_n[s0][0][e0], ncy = s_n[s0][1][e0], ncz = s_n[s0][2][e0];
@@ -164,44 +196,39 @@ __global__ __launch_bounds__(TX * TY) void k_collide(CollideArgs a) {
         float fv[Q];
         for_links([&](auto lc) {
             constexpr int l = decltype(lc)::value;
-            const float* __restrict__ fl = a.fsrc + (size_t)l * G.plane;
-            fv[l] = fl[zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]];
+            fv[l] = fsrc.ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
         });
         const Moments m = moments(fv, a.phys, Fx, Fy, Fz);
 
         if constexpr (MODE == COLLIDE_MACROS) {
-            a.rho[me] = m.rho;
-            a.ux[me] = m.ux;
-            a.uy[me] = m.uy;
-            a.uz[me] = m.uz;
-            a.fx[me] = Fx;
-            a.fy[me] = Fy;
-            a.fz[me] = Fz;
-            a.nx[me] = ncx;
-            a.ny[me] = ncy;
-            a.nz[me] = ncz;
+            st32(a.rho, me, m.rho);
+            st32(a.ux, me, m.ux);
+            st32(a.uy, me, m.uy);
+            st32(a.uz, me, m.uz);
+            st32(a.fx, me, Fx);
+            st32(a.fy, me, Fy);
+            st32(a.fz, me, Fz);
+            st32(a.nx, me, ncx);
+            st32(a.ny, me, ncy);
+            st32(a.nz, me, ncz);
         } else {
             const FluidCell fc = fluid_cell(m, Fx, Fy, Fz);
             for_links([&](auto lc) {
                 constexpr int l = decltype(lc)::value;
-                float* __restrict__ fl = a.fdst + (size_t)l * G.plane;
-                fl[me] = fluid_post<l>(fc, a.phys, d);
+                fdst.st<l>(me, fluid_post<l>(fc, a.phys, d));
             });
             const float phic = s_phi[z & 3][(ty + 2) * PX + (tx + 2)];
             const PhaseCell pc = phase_cell(a.phys, phic, m.ux, m.uy, m.uz, ncx, ncy, ncz);
             if constexpr (MODE == COLLIDE_STEP) {
                 for_links([&](auto lc) {
                     constexpr int l = decltype(lc)::value;
-                    float* __restrict__ gl = a.gdst + (size_t)l * G.plane;
-                    gl[me] = phase_post<l>(pc, a.phys);
+                    gdst.st<l>(me, phase_post<l>(pc, a.phys));
                 });
             } else {
                 for_links([&](auto lc) {
                     constexpr int l = decltype(lc)::value;
-                    const float* __restrict__ gs = a.gsrc + (size_t)l * G.plane;
-                    float* __restrict__ gl = a.gdst + (size_t)l * G.plane;
-                    const float gpre = gs[zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]];
-                    gl[me] = phase_post_bgk<l>(pc, a.phys, d, gpre);
+                    const float gpre = gsrc.ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+                    gdst.st<l>(me, phase_post_bgk<l>(pc, a.phys, d, gpre));
                 });
             }
         }
@@ -309,7 +336,9 @@ const Tiling kTilings[] = {
 
 template <int TX, int TY>
 hipError_t launch_collide_t(const CollideArgs& a, int mode, hipStream_t s) {
-    dim3 grid((a.geo.nx + TX - 1) / TX, (a.geo.ny + TY - 1) / TY, (a.geo.nz + a.zc - 1) / a.zc);
+    const unsigned tiles = (unsigned)((a.geo.nx + TX - 1) / TX) * (unsigned)((a.geo.ny + TY - 1) / TY) *
+                           (unsigned)((a.geo.nz + a.zc - 1) / a.zc);
+    dim3 grid(tiles, 1, 1);
     dim3 block(TX, TY, 1);
     switch (mode) {
         case COLLIDE_STEP: hipLaunchKernelGGL((k_collide<TX, TY, COLLIDE_STEP>), grid, block, 0, s, a); break;
@@ -332,7 +361,8 @@ int tiling_count() { return (int)(sizeof(kTilings) / sizeof(kTilings[0])); }
 Tiling tiling_at(int i) { return kTilings[i]; }
 
 hipError_t launch_phase(const float* g, float* phi, const Geo& geo, hipStream_t s) {
-    hipLaunchKernelGGL(k_phase, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, g, phi, geo);
+    const unsigned gx = (unsigned)(geo.nx + PBX - 1) / PBX, gy = (unsigned)(geo.ny + PBY - 1) / PBY;
+    hipLaunchKernelGGL(k_phase, dim3(gx * gy * (unsigned)geo.nz), dim3(PBX, PBY, 1), 0, s, g, phi, geo, gx, gy);
     return hipGetLastError();
 }
 

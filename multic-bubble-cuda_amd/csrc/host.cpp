@@ -23,7 +23,8 @@ mcb_status host_validate(const mcb_params* p, std::string* why) {
     };
     if (!p) return bad("null parameters");
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return bad("every side must be >= 3");
-    if ((long long)p->nx * p->ny * p->nz >= (1LL << 31)) return bad("lattice exceeds 2^31 cells");
+    // Kernels address a population plane with 32-bit byte offsets.
+    if ((long long)p->nx * p->ny * p->nz >= (1LL << 30)) return bad("lattice exceeds 2^30 cells");
     if (p->nz > 65535 || p->ny > 65535) return bad("ny and nz must be <= 65535");
     if (p->bc_mode != MCB_BC_PERIODIC && p->bc_mode != MCB_BC_REFERENCE) return bad("unknown bc_mode");
     if (p->weight_set != MCB_WEIGHTS_LITERAL && p->weight_set != MCB_WEIGHTS_UNIT_SUM) return bad("unknown weight_set");

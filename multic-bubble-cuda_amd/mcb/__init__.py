@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "libmcb.so")
+# MCB_LIBRARY selects another build of libmcb (e.g. a variant for an A/B timing).
+LIB_PATH = os.environ.get("MCB_LIBRARY") or os.path.join(_PKG_DIR, "libmcb.so")
 DRIVER_PATH = os.path.join(_PKG_DIR, "bin", "mcb")
 
 # mcb_status
