@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only time a plain device copy of BYTES (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -69,6 +71,11 @@ def main():
         sys.exit(2)
 
     import mcb
+    if args.copy_probe:
+        ms = mcb.bench_copy(args.copy_probe, 3)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms,
+                          "gbs": 2 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
     cells = p.nx * p.ny * p.nz

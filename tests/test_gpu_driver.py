@@ -1,0 +1,106 @@
+"""The mcb driver end to end, and failure handling of the context API.
+
+The driver keeps the reference's surface (src/main.cu): `mcb <VS> <ID>`
+writes <root>bin/<VS>/<ID>/<ID>_info.txt and <ID>_phi<STEP:06d>.bin after
+iteration STEP for STEP % save == 0. The reference preset runs the staged
+path with ghost-layer boundaries; its files must equal the oracle's d_phi
+bit for bit and parse under the post-processing rules.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import postfmt
+from conftest import f32_bubbles, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def test_driver_reference_preset_matches_oracle(gpu_lib, tmp_path):
+    mcb = gpu_lib
+    root = str(tmp_path) + "/"
+    r = subprocess.run([mcb.DRIVER_PATH, "D3Q19", "000", "1", "--preset", "reference", "--steps", "200",
+                        "--root", root], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sim = root + "bin/D3Q19/000/"
+    info = postfmt.sim_info(sim)
+    assert (info["NX"], info["NY"], info["NZ_TOTAL"], info["Nsteps"]) == (64, 64, 256, 200)
+    assert postfmt.macr_steps(sim) == [0, 100, 200]
+    c = mcb.case_preset("reference")
+    o = oracle_for(c.params)
+    o.init(f32_bubbles(mcb.case_bubbles(c)), c.width)
+    done = 0
+    for step in (0, 100, 200):
+        o.step(step + 1 - done)        # the file of STEP holds d_phi after iteration STEP
+        done = step + 1
+        got = postfmt.read_macr(postfmt.macr_file(sim, "000", "phi", step), sim)
+        want = np.swapaxes(o.field("phi"), 0, 2)
+        assert np.array_equal(got, want), f"step {step}"
+
+
+def test_driver_periodic_preset_with_extra_fields(gpu_lib, tmp_path):
+    mcb = gpu_lib
+    root = str(tmp_path) + "/"
+    r = subprocess.run([mcb.DRIVER_PATH, "D3Q19", "cfg0", "--preset", "cfg0", "--steps", "50", "--save-every",
+                        "25", "--extra-fields", "--root", root], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sim = root + "bin/D3Q19/cfg0/"
+    assert postfmt.macr_steps(sim) == [0, 25, 50]
+    for name in ("rho", "ux", "uy", "uz"):
+        assert os.path.exists(postfmt.macr_file(sim, "cfg0", name, 50))
+    rho = postfmt.read_macr(postfmt.macr_file(sim, "cfg0", "rho", 50), sim)
+    assert np.all(np.isfinite(rho)) and abs(rho.mean() - 1) < 1e-2
+
+
+def test_state_errors_are_statuses(gpu_lib):
+    mcb = gpu_lib
+    lat = mcb.Lattice(mcb.make_params(16, 16, 16))
+    with pytest.raises(mcb.McbError) as e:
+        lat.step(1)                       # no initial state yet
+    assert e.value.status == mcb.ERR_INVALID_ARG
+    lat.init_bubbles([(8, 8, 8, 4)])
+    with pytest.raises(mcb.McbError):
+        lat.write("phi", np.zeros(10, np.float32))
+    with pytest.raises(mcb.McbError):
+        lat.init_phi(np.zeros(7, np.float32))
+    with pytest.raises(mcb.McbError):
+        lat.set_tiling(48, 3, 4)
+    lat.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_nonfinite_populations_are_detected(gpu_lib, path):
+    mcb = gpu_lib
+    p = mcb.make_params(16, 16, 16, tau_f=0.6, path=mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED)
+    lat = mcb.Lattice(p)
+    lat.init_bubbles([(8, 8, 8, 4)])
+    lat.step(2)
+    lat.compute_macros()
+    s, ke, ok = lat.reduce()
+    assert ok and np.isfinite(s)
+    g = lat.read("g")
+    g[3, 4, 5, 6] = np.nan               # fault injection through the ABI
+    lat.write("g", g)
+    lat.step(2)
+    lat.compute_macros()
+    s, ke, ok = lat.reduce()
+    assert not ok and not np.isfinite(s)
+    lat.close()
+
+
+def test_cfg1_geometry_bitexact_short(gpu_lib):
+    """The 128^3 seeded off-centre case, 20 steps, fused path vs oracle."""
+    mcb = gpu_lib
+    c = mcb.case_preset("cfg1", 0)
+    lat = mcb.Lattice.from_case(c)
+    o = oracle_for(c.params)
+    o.init(f32_bubbles(mcb.case_bubbles(c)), c.width)
+    lat.step(20)
+    o.step(20)
+    got = lat.observables()
+    o.macros()
+    for k in ("phi", "rho", "ux", "uy", "uz"):
+        assert np.array_equal(got[k], o.field(k)), k
+    lat.close()

@@ -1,0 +1,409 @@
+"""Known-answer and invariant checks that pin the oracle stage by stage.
+
+The reference holds no golden vectors or tests, so each transcribed kernel
+is pinned by what the mathematics fixes: closed forms of the stencils,
+conservation laws of the collisions, exact permutations of streaming, and
+physical laws of the whole step. Each check is chosen so that a plausible
+transcription slip (a dropped term, a wrong sign or index, a transposed
+operand) fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+C = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0],
+              [0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1],
+              [0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1]])
+CS2 = 1.0 / 3.0
+
+
+def grid(shape):
+    nz, ny, nx = shape
+    return np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+
+
+# ---------------------------------------------------------------- phiCalc
+
+def test_phi_calc_is_sequential_float_sum():
+    o = O.Oracle(5, 4, 3, prec=32)
+    rng = np.random.default_rng(1)
+    g = rng.uniform(0, 0.1, (19, 3, 4, 5)).astype(np.float32)
+    o.field("g")[...] = g
+    o.stage("phiCalc")
+    want = np.zeros((3, 4, 5), np.float32)
+    for l in range(19):           # lbm.cu:19-23: sum += g[l] in link order
+        want = (want + g[l]).astype(np.float32)
+    assert np.array_equal(o.field("phi"), want)
+
+
+def test_phi_of_initial_state_is_initial_phi():
+    o = O.Oracle(24, 24, 24, prec=64)
+    o.init([(12, 12, 12, 6)], 3.0)
+    phi0 = o.field("phi").copy()
+    o.stage("phiCalc")
+    assert np.allclose(o.field("phi"), phi0, rtol=0, atol=1e-15)
+
+
+def test_initial_profile_matches_init_cu():
+    # init.cu:39-46: phi = 0.5 + 0.5 tanh(2 (R - r) / 3), centre N/2
+    o = O.Oracle(32, 32, 32, prec=64, bc_mode=O.REFERENCE)
+    o.init([(16, 16, 16, 8)], 3.0)
+    z, y, x = grid((32, 32, 32))
+    r = np.sqrt((x - 16.0) ** 2 + (y - 16.0) ** 2 + (z - 16.0) ** 2)
+    want = 0.5 + 0.5 * np.tanh(2 * (8 - r) / 3)
+    interior = (x > 0) & (x < 31) & (y > 0) & (y < 31) & (z > 0) & (z < 31)
+    phi = o.field("phi")
+    assert np.allclose(phi[interior], want[interior], atol=1e-15)
+    assert np.all(phi[~interior] == 0)            # faces keep the memset zero
+    assert np.all(o.field("rho") == 1)
+    f = o.field("f")
+    assert np.allclose(f, W.reshape(19, 1, 1, 1) * 1.0)
+
+
+# --------------------------------------------------------------- gradCalc
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_gradient_of_a_sine_is_sin_k_cos_kx(axis):
+    """For phi(x) = sin(k x): G = sum 3 w c phi(x+c) = (phi(x+1)-phi(x-1))/2 = sin k cos kx."""
+    n = 16
+    o = O.Oracle(n, n, n, prec=64)
+    coords = grid((n, n, n))[::-1]          # x, y, z
+    k = 2 * np.pi * 3 / n
+    o.field("phi")[...] = np.sin(k * coords[axis])
+    o.stage("gradCalc")
+    g = np.sin(k) * np.cos(k * coords[axis])
+    big = np.abs(g) > 1e-3
+    comps = [o.field("normx"), o.field("normy"), o.field("normz")]
+    for a in range(3):
+        if a == axis:
+            assert np.allclose(comps[a][big], np.sign(g[big]), rtol=0, atol=1e-14)
+        else:
+            # zero in exact arithmetic; the sequential link sum leaves rounding residue
+            assert np.allclose(comps[a], 0, atol=1e-14)
+    # indicator = |G|^2 / sqrt(max(|G|^2, 1e-9)) = |G| where |G| is not tiny
+    assert np.allclose(o.field("indicator")[big], np.abs(g[big]), rtol=0, atol=1e-14)
+
+
+def test_gradient_of_separable_sum_has_no_cross_terms():
+    n = 12
+    o = O.Oracle(n, n, n, prec=64)
+    z, y, x = grid((n, n, n))
+    k = 2 * np.pi / n
+    o.field("phi")[...] = np.sin(k * x) + 2 * np.sin(2 * k * y) - np.sin(3 * k * z)
+    o.stage("gradCalc")
+    g = np.stack([np.sin(k) * np.cos(k * x), 2 * np.sin(2 * k) * np.cos(2 * k * y),
+                  -np.sin(3 * k) * np.cos(3 * k * z)])
+    mag = np.sqrt(np.maximum((g ** 2).sum(0), 1e-9))
+    for a, name in enumerate(("normx", "normy", "normz")):
+        assert np.allclose(o.field(name), g[a] / mag, atol=1e-12)
+
+
+def test_planar_tanh_interface_has_x_normal():
+    o = O.Oracle(32, 8, 8, prec=32)
+    z, y, x = grid((8, 8, 32))
+    o.field("phi")[...] = 0.5 + 0.5 * np.tanh(2 * (10.3 - x) / 3)
+    o.stage("gradCalc")
+    nx = o.field("normx")
+    assert np.allclose(o.field("normy"), 0, atol=1e-6) and np.allclose(o.field("normz"), 0, atol=1e-6)
+    band = (x > 5) & (x < 15)
+    assert np.allclose(nx[band], -1.0, rtol=0, atol=1e-6)
+
+
+# ---------------------------------------------------------- curvatureCalc
+
+def test_flat_interface_has_zero_curvature():
+    o = O.Oracle(16, 16, 16, prec=64)
+    for name, v in (("normx", 0.6), ("normy", 0.0), ("normz", 0.8)):
+        o.field(name)[...] = v
+    o.field("indicator")[...] = 0.3
+    o.stage("curvatureCalc")
+    assert np.allclose(o.field("curvature"), 0.0, atol=1e-15)
+    assert np.allclose(o.field("ffx"), 0.0, atol=1e-15)
+
+
+def test_radial_normal_has_curvature_minus_two_over_r():
+    """kappa = -div n; for n = r_hat, div n = 2/r in 3-D."""
+    n = 48
+    o = O.Oracle(n, n, n, prec=64, sigma=0.1)
+    z, y, x = grid((n, n, n))
+    dx, dy, dz = x - 24.0, y - 24.0, z - 24.0
+    r = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2)
+    r = np.where(r == 0, 1, r)
+    o.field("normx")[...] = dx / r
+    o.field("normy")[...] = dy / r
+    o.field("normz")[...] = dz / r
+    o.field("indicator")[...] = 0.5
+    o.stage("curvatureCalc")
+    shell = (r > 9) & (r < 15)
+    kappa = o.field("curvature")[shell]
+    assert np.allclose(kappa, -2.0 / r[shell], rtol=0.03)
+    # F = sigma * kappa * n * ind
+    assert np.allclose(o.field("ffx")[shell], 0.1 * kappa * (dx / r)[shell] * 0.5, rtol=1e-12)
+
+
+# ---------------------------------------------- momentiCalc, collisionFluid
+
+def feq_he(rho, u, F):
+    """Equilibrium with He forcing, as momentiCalc builds it (FP64)."""
+    cu = np.einsum("ad,a...->d...", C, u) / CS2
+    uu = 0.5 * (u ** 2).sum(0) / CS2
+    common = W.reshape(19, 1, 1, 1) * rho * (1 + cu + 0.5 * cu ** 2 - uu)
+    cmu = C.reshape(3, 19, 1, 1, 1) - u[:, None]
+    hef = 0.5 * common * np.einsum("a...,a...->...", cmu, F[:, None]) / (rho * CS2)
+    return common - hef
+
+
+def test_rest_state_is_a_fixed_point():
+    """f = w, F = 0: u is exactly 0 (opposite links cancel); rho and Pi are 1 and 0
+    up to the rounding of the float weight sum; one collision returns f."""
+    for prec, tol in ((32, 1e-6), (64, 1e-15)):
+        o = O.Oracle(6, 5, 4, prec=prec, tau_f=0.55)
+        o.init([(3, 3, 2, 0.0)], 3.0)      # R = 0: f = w everywhere, forces zero
+        f0 = o.field("f").copy()
+        o.stage("momentiCalc")
+        assert np.allclose(o.field("rho"), 1, rtol=0, atol=tol)
+        for k in ("ux", "uy", "uz", "pxy", "pxz", "pyz"):
+            assert np.all(o.field(k) == 0), k
+        for k in ("pxx", "pyy", "pzz"):
+            assert np.allclose(o.field(k), 0, atol=tol), k
+        o.stage("collisionFluid")
+        assert np.allclose(o.field("f"), f0, rtol=tol, atol=0)
+
+
+def test_moments_recover_rho_u_and_zero_pi_at_equilibrium():
+    rng = np.random.default_rng(3)
+    shape = (4, 5, 6)
+    o = O.Oracle(6, 5, 4, prec=64)
+    rho = 1 + 0.05 * rng.standard_normal(shape)
+    u = 0.02 * rng.standard_normal((3,) + shape)
+    F = np.zeros((3,) + shape)
+    o.field("f")[...] = feq_he(rho, u, F)
+    o.stage("momentiCalc")
+    assert np.allclose(o.field("rho"), rho, atol=1e-14)
+    for a, k in enumerate(("ux", "uy", "uz")):
+        assert np.allclose(o.field(k), u[a], atol=1e-14)
+    for k in ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"):
+        assert np.allclose(o.field(k), 0, atol=1e-14), k
+
+
+def test_fluid_collision_conserves_mass_and_adds_the_force():
+    """sum f* = rho; sum c f* = sum c f + F (He forcing with u = j/rho + F/2rho)."""
+    rng = np.random.default_rng(4)
+    shape = (5, 6, 7)
+    o = O.Oracle(7, 6, 5, prec=64, tau_f=0.6)
+    f = W.reshape(19, 1, 1, 1) * (1 + 0.02 * rng.standard_normal((19,) + shape))
+    F = 1e-3 * rng.standard_normal((3,) + shape)
+    o.field("f")[...] = f
+    for a, k in enumerate(("ffx", "ffy", "ffz")):
+        o.field(k)[...] = F[a]
+    o.stage("momentiCalc")
+    rho = o.field("rho").copy()
+    o.stage("collisionFluid")       # pushes f*_l(x) to x + c_l
+    fs = o.field("f")
+    # undo the push: f*_l(x) = f_l(x + c_l)
+    post = np.stack([np.roll(fs[l], shift=(-C[2, l], -C[1, l], -C[0, l]), axis=(0, 1, 2)) for l in range(19)])
+    assert np.allclose(post.sum(0), rho, atol=1e-13)
+    for a in range(3):
+        want = np.einsum("l,l...->...", C[a], f) + F[a]
+        assert np.allclose(np.einsum("l,l...->...", C[a], post), want, atol=1e-13)
+
+
+# ---------------------------------------------------------- collisionPhase
+
+def test_phase_collision_moments():
+    """sum g* = phi; sum c g* = phi u + cs^2 SHARP_C phi (1 - phi) n (tau_g = 1)."""
+    rng = np.random.default_rng(5)
+    shape = (4, 5, 6)
+    o = O.Oracle(6, 5, 4, prec=64, tau_g=1.0)
+    phi = rng.uniform(0, 1, shape)
+    u = 0.01 * rng.standard_normal((3,) + shape)
+    n = rng.standard_normal((3,) + shape)
+    n /= np.linalg.norm(n, axis=0)
+    o.field("phi")[...] = phi
+    for a, (ku, kn) in enumerate((("ux", "normx"), ("uy", "normy"), ("uz", "normz"))):
+        o.field(ku)[...] = u[a]
+        o.field(kn)[...] = n[a]
+    o.stage("collisionPhase")
+    g = o.field("g")
+    s = 0.15 * 3.0
+    assert np.allclose(g.sum(0), phi, atol=1e-14)
+    for a in range(3):
+        want = phi * u[a] + CS2 * s * phi * (1 - phi) * n[a]
+        assert np.allclose(np.einsum("l,l...->...", C[a], g), want, atol=1e-14)
+
+
+def test_phase_bgk_conserves_phi_and_reduces_to_reference_at_tau_one():
+    rng = np.random.default_rng(6)
+    shape = (3, 4, 5)
+    state = {k: rng.uniform(0.0, 1.0, shape) for k in ("phi",)}
+    outs = {}
+    for tau_g in (1.0, 0.8):
+        o = O.Oracle(5, 4, 3, prec=64, tau_g=tau_g)
+        o.field("phi")[...] = state["phi"]
+        for k in ("ux", "uy", "uz"):
+            o.field(k)[...] = 0.003
+        for k in ("normx", "normy", "normz"):
+            o.field(k)[...] = 1 / np.sqrt(3)
+        g0 = W.reshape(19, 1, 1, 1) * rng.uniform(0, 1, (19,) + shape)
+        o.field("g")[...] = g0
+        o.stage("collisionPhase")
+        outs[tau_g] = (o.field("g").copy(), g0)
+    g1, _ = outs[1.0]
+    g08, g0 = outs[0.8]
+    # BGK: g* = (1 - w) g + w g1 with w = 1/0.8
+    assert np.allclose(g08, (1 - 1.25) * g0 + 1.25 * g1, atol=1e-14)
+    assert np.allclose(g08.sum(0), (1 - 1.25) * g0.sum(0) + 1.25 * state["phi"], atol=1e-14)
+
+
+# ---------------------------------------------------------- streamingCalc
+
+def test_streaming_is_the_link_shift_permutation():
+    o = O.Oracle(5, 4, 3, prec=32)
+    g = np.zeros((19, 3, 4, 5), np.float32)
+    for l in range(19):
+        g[l, 1, 2, 3] = l + 1
+    o.field("g")[...] = g
+    o.stage("streamingCalc")
+    out = o.field("g")      # swapped in: the streamed buffer
+    for l in range(19):
+        nz_, ny_, nx_ = np.argwhere(out[l] != 0)[0]
+        assert out[l].sum() == l + 1
+        assert (nx_, ny_, nz_) == ((3 + C[0, l]) % 5, (2 + C[1, l]) % 4, (1 + C[2, l]) % 3)
+
+
+# --------------------------------------------------------- reference mode
+
+def test_boundary_conditions_clamp_faces_serially():
+    o = O.Oracle(5, 6, 7, prec=32, bc_mode=O.REFERENCE)
+    rng = np.random.default_rng(8)
+    phi = rng.uniform(size=(7, 6, 5)).astype(np.float32)
+    o.field("phi")[...] = phi
+    o.stage("boundaryConditions")
+    z, y, x = grid((7, 6, 5))
+    want = phi[np.clip(z, 1, 5), np.clip(y, 1, 4), np.clip(x, 1, 3)]
+    assert np.array_equal(o.field("phi"), want)
+
+
+def test_moving_wall_has_no_observable_effect():
+    """movingWallBC (lbm.cu:430-461) writes f on j faces, which nothing reads."""
+    res = []
+    for mw in (False, True):
+        o = O.Oracle(16, 16, 24, prec=32, bc_mode=O.REFERENCE, weights=O.W_LITERAL, tau_f=0.505, moving_wall=mw)
+        o.init([(8, 8, 12, 5)], 3.0)
+        o.step(12)
+        o.macros()
+        res.append({k: o.field(k).copy() for k in ("phi", "rho", "ux", "uy", "uz")})
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
+def test_reference_mode_equals_periodic_away_from_faces_after_one_step():
+    """The two modes differ only through face values; after one step plus the
+    macros of the next, a cell depends on phi_0 within 5 cells (curvature of
+    normals of phi_1, itself streamed from collisions that read the force)."""
+    outs = []
+    for bc in (O.REFERENCE, O.PERIODIC):
+        o = O.Oracle(20, 20, 20, prec=64, bc_mode=bc, tau_f=0.55)
+        o.init([(10, 10, 10, 4)], 3.0)
+        o.step(1)
+        o.macros()
+        outs.append({k: o.field(k).copy() for k in ("phi", "rho", "ux")})
+    core = (slice(6, 14),) * 3
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k][core], outs[1][k][core]), k
+
+
+# ------------------------------------------------------------- whole step
+
+def test_openmp_is_bitwise_equal_to_serial():
+    res = []
+    for threads in (1, 4):
+        O.set_threads(threads)
+        o = O.Oracle(20, 18, 16, prec=32, tau_f=0.55, tau_g=0.8)
+        o.init([(10.2, 9.1, 8.3, 5)], 3.0)
+        o.step(5)
+        o.macros()
+        res.append({k: o.field(k).copy() for k in ("phi", "rho", "ux", "uy", "uz")})
+    O.set_threads(8)
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
+def test_periodic_step_conserves_phi():
+    o = O.Oracle(32, 32, 32, prec=32, tau_f=0.55, tau_g=0.8, weights=O.W_UNIT_SUM)
+    o.init([(16, 16, 16, 7)], 3.0)
+    o.macros()
+    s0 = o.sum_phi()
+    o.step(200)
+    o.macros()
+    assert abs(o.sum_phi() - s0) / s0 < 1e-7
+
+
+def test_literal_weights_gain_mass_at_two_pow_minus_26_per_step():
+    """The reference's float weights sum to 1 + 2^-26 (var.cu:40-45)."""
+    o = O.Oracle(24, 24, 24, prec=32, tau_f=0.55, tau_g=1.0, weights=O.W_LITERAL)
+    o.init([(12, 12, 12, 6)], 3.0)
+    o.macros()
+    s0 = o.sum_phi()
+    o.step(100)
+    o.macros()
+    per_step = (o.sum_phi() / s0 - 1) / 100
+    assert 0.8 * 2 ** -26 < per_step < 1.3 * 2 ** -26
+
+
+def test_centred_bubble_keeps_its_symmetries():
+    """Mirror i -> 24 - i and the x <-> y swap hold in exact arithmetic. In floating
+    point they hold to rounding for the first steps only: where |grad phi| is
+    near zero (inside the bubble) the normal grad phi / |grad phi| takes its
+    direction from rounding noise, and the sharpening flux phi (1 - phi) n then
+    amplifies differences about tenfold per step (a property of the
+    reference's normal, src/lbm.cu:65-71). A sign or index slip shows at once."""
+    o = O.Oracle(24, 24, 24, prec=64, tau_f=0.6, tau_g=0.8)
+    o.init([(12, 12, 12, 6)], 3.0)
+    o.step(3)
+    o.macros()
+    phi, ux, uy = o.field("phi"), o.field("ux"), o.field("uy")
+    m = (24 - np.arange(24)) % 24
+    assert np.allclose(phi, phi[:, :, m], atol=1e-14)
+    assert np.allclose(phi, phi.transpose(0, 2, 1), atol=1e-14)
+    assert np.allclose(ux, -ux[:, :, m], atol=1e-14)
+    assert np.allclose(ux, uy.transpose(0, 2, 1), atol=1e-14)
+    assert np.abs(ux).max() > 1e-3      # the flow is not trivially zero
+
+
+def test_fp32_tracks_fp64():
+    outs = {}
+    for prec in (32, 64):
+        o = O.Oracle(24, 24, 24, prec=prec, tau_f=0.55, tau_g=0.8)
+        o.init([(12.3, 11.7, 12.1, 6)], 3.0)
+        o.step(100)
+        o.macros()
+        outs[prec] = {k: o.field(k).astype(np.float64) for k in ("phi", "rho", "ux")}
+    for k, tol in (("phi", 1e-5), ("rho", 1e-6), ("ux", 1e-3)):
+        a, b = outs[32][k], outs[64][k]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < tol, k
+
+
+def _delta_p(o, c, R, width=3.0):
+    rho = o.field("rho").astype(np.float64)
+    nz, ny, nx = rho.shape
+    z, y, x = grid(rho.shape)
+    mi = lambda d, n: (d + n / 2) % n - n / 2
+    r = np.sqrt(mi(x - c[0], nx) ** 2 + mi(y - c[1], ny) ** 2 + mi(z - c[2], nz) ** 2)
+    return CS2 * (rho[r < R - 2 * width].mean() - rho[r > R + 2 * width].mean())
+
+
+def test_laplace_pressure_jump():
+    """Near equilibrium the pressure jump approaches 2 sigma / R (3-D Laplace law)."""
+    n, R, sigma = 48, 10.0, 0.1
+    res = {}
+    for prec in (32, 64):
+        o = O.Oracle(n, n, n, prec=prec, tau_f=1.0, tau_g=0.8, sigma=sigma)
+        o.init([(24, 24, 24, R)], 3.0)
+        o.step(800)
+        o.macros()
+        res[prec] = _delta_p(o, (24, 24, 24), R)
+    assert abs(res[64] / (2 * sigma / R) - 1) < 0.15
+    assert abs(res[32] / res[64] - 1) < 1e-3
