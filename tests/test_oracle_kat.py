@@ -373,6 +373,28 @@ def test_centred_bubble_keeps_its_symmetries():
     assert np.abs(ux).max() > 1e-3      # the flow is not trivially zero
 
 
+def test_one_ulp_perturbation_grows_beyond_the_parity_tolerance():
+    """Why the kernels are bit-exact: a 1-ulp change of the initial populations
+    grows past the 1e-5 relative-L2 parity tolerance within 200 steps (1.8% in
+    phi and ~10% in u after 1000 steps of the 64^3 case), so an implementation
+    that rounds differently cannot meet that tolerance at 1000 steps."""
+    res = []
+    for pert in (False, True):
+        o = O.Oracle(32, 32, 32, prec=32, tau_f=0.55, tau_g=0.8, sigma=0.1)
+        o.init([(16, 16, 16, 8)], 3.0)
+        if pert:
+            g = o.field("g")
+            d = np.random.default_rng(0).integers(-1, 2, size=g.shape)
+            up, dn = np.nextafter(g, np.float32(np.inf)), np.nextafter(g, np.float32(-np.inf))
+            g[...] = np.where((d > 0) & (g != 0), up, np.where((d < 0) & (g != 0), dn, g))
+        o.step(200)
+        o.macros()
+        res.append({k: o.field(k).astype(np.float64) for k in ("phi", "ux")})
+    for k in ("phi", "ux"):
+        a, b = res
+        assert np.linalg.norm(a[k] - b[k]) / np.linalg.norm(a[k]) > 1e-3, k
+
+
 def test_fp32_tracks_fp64():
     outs = {}
     for prec in (32, 64):
