@@ -147,6 +147,41 @@ mcb_status mcb_get_tiling(const mcb_ctx* ctx, int* tx, int* ty, int* zc);
 int mcb_tiling_count(void);
 mcb_status mcb_tiling_at(int i, int* tx, int* ty);
 
+/* Time every supported tile shape and z-chunk on a scratch lattice with these
+ * parameters (on p->device) and remember the fastest; later contexts and
+ * batch cases of the same sides and storage use it. No reference counterpart
+ * (the reference launches fixed 8x8x8 blocks, src/main.cu:43-46). */
+mcb_status mcb_autotune(const mcb_params* p, int* tx, int* ty, int* zc, double* ms_per_step);
+
+/* ---- Batches: many independent periodic cases resident on one device ----
+ * A batch step advances every case by one step with one launch per pass for
+ * all cases (grouped by tile shape); two steps at a time are replayed from a
+ * captured hipGraph. All cases of a batch share one storage form. Cases never
+ * exchange data: a case's results do not depend on the rest of the batch.
+ * No reference counterpart (the reference runs one lattice per process). */
+typedef struct mcb_batch mcb_batch;
+/* Device bytes one case occupies in a batch (populations and anchors). */
+size_t mcb_case_bytes(const mcb_params* p);
+mcb_status mcb_batch_create(int device, int storage, void* stream, mcb_batch** out);
+mcb_status mcb_batch_destroy(mcb_batch* b);
+/* Add a case with the initial state of mcb_init_bubbles / mcb_init_phi. */
+mcb_status mcb_batch_add(mcb_batch* b, const mcb_params* p, int n, const mcb_bubble* bubbles, float width,
+                         int* case_id);
+mcb_status mcb_batch_add_phi(mcb_batch* b, const mcb_params* p, const float* phi, size_t n, int* case_id);
+mcb_status mcb_batch_remove(mcb_batch* b, int case_id);
+mcb_status mcb_batch_step(mcb_batch* b, int nsteps);
+mcb_status mcb_batch_step_timed(mcb_batch* b, int nsteps, double* ms);
+mcb_status mcb_batch_compute_macros(mcb_batch* b, int case_id);
+mcb_status mcb_batch_read_field(mcb_batch* b, int case_id, int field, float* host, size_t n);
+mcb_status mcb_batch_write_field(mcb_batch* b, int case_id, int field, const float* host, size_t n);
+mcb_status mcb_batch_reduce(mcb_batch* b, int case_id, double* sum_phi, double* kinetic);
+mcb_status mcb_batch_case_steps(mcb_batch* b, int case_id, long long* steps);
+mcb_status mcb_batch_set_tiling(mcb_batch* b, int case_id, int tx, int ty, int zc);
+mcb_status mcb_batch_synchronize(mcb_batch* b);
+int mcb_batch_count(const mcb_batch* b);
+size_t mcb_batch_bytes(const mcb_batch* b);
+const char* mcb_batch_last_error(const mcb_batch* b);
+
 int mcb_abi_version(void);
 const char* mcb_last_error(const mcb_ctx* ctx);
 const char* mcb_status_string(mcb_status s);
@@ -170,10 +205,68 @@ typedef struct mcb_case {
  * (64x64x256, ghost-layer boundaries, tau 0.505, one R=20 bubble at the
  * centre, 25000 steps, phi saved every 100). "cfg0" .. "cfg2": the periodic
  * benchmark cases (64^3 R=10 centred; 128^3 seeded R in [12,24] off-centre;
- * 256^3 two seeded non-overlapping bubbles R in [16,32]). Random draws use
- * splitmix64(seed) mapped to [0,1) by the top 53 bits, so every consumer of
- * a (name, seed) pair gets the same geometry. */
+ * 256^3 two seeded non-overlapping bubbles R in [16,32]). "batch": one case
+ * of the uneven batch (cube side in {32,48,64,96,128}, R in [0.15,0.3] side,
+ * sigma in [0.01,0.1], tau_f in [0.52,0.8], tau_g 1, FP32, 2000 steps).
+ * "serve": one serving request (side a multiple of 16 in [48,192], R in
+ * [0.1,0.3] side, sigma in [0.01,0.1], tau_f in [0.52,0.8], tau_g 1, steps
+ * log-uniform in [500,5000], FP16 storage). Bubbles of "batch" and "serve"
+ * sit at a seeded position. Random draws use splitmix64(seed) mapped to
+ * [0,1) by the top 53 bits, so every consumer of a (name, seed) pair gets
+ * the same case; "batch" and "serve" are meant to be called with seeds
+ * base + i for case i. */
 mcb_status mcb_case_preset(const char* name, uint64_t seed, mcb_case* out);
+
+/* ---- Serving: bubble cases arriving over time, continuously batched ----
+ * No reference counterpart (the reference runs one case per process). */
+typedef struct mcb_request {
+    mcb_case c;             /* parameters (periodic), bubbles, width; c.nsteps steps are run */
+    double arrival_s;       /* arrival time, seconds after the start of the run */
+    float* phi_out;         /* optional host buffer of nx*ny*nz floats for the final phi */
+} mcb_request;
+
+typedef struct mcb_request_result {
+    int status;             /* mcb_status of this request */
+    int reserved;
+    double arrival_s, admit_s, finish_s;   /* seconds after the start of the run */
+    double sum_phi, kinetic;               /* FP64 diagnostics of the final state */
+    long long lattice_updates;             /* cells * steps */
+} mcb_request_result;
+
+typedef struct mcb_serve_config {
+    int device;
+    int storage;            /* population storage of every request (MCB_STORE_*) */
+    size_t hbm_budget;      /* device bytes the resident cases may hold; 0 = 90% of free memory */
+    int max_cases;          /* resident cases at most; 0 = no limit */
+    int tick_steps;         /* steps between scheduling decisions at most; 0 = 100 */
+    int sequential;         /* 1: no batching, one request at a time (the baseline) */
+    int reserved[7];
+} mcb_serve_config;
+
+typedef struct mcb_serve_stats {
+    double makespan_s;      /* first arrival to last result */
+    double lattice_updates; /* sum over completed requests */
+    double mlups;           /* lattice_updates / makespan */
+    double p50_s, p99_s, mean_s, max_s;    /* time to result (finish - arrival) */
+    int completed, rejected;
+    int peak_cases, rounds; /* most resident cases; scheduling rounds */
+    double peak_bytes;      /* most device bytes held by resident cases */
+} mcb_serve_stats;
+
+/* Replay a request trace in real time on one device. A request is admitted
+ * once it has arrived, in arrival order, when its footprint (mcb_case_bytes)
+ * fits the budget next to the resident cases; resident cases advance together
+ * as one batch, and new cases join between scheduling rounds while others
+ * run. A round advances min(tick_steps, fewest steps left) steps, so a case
+ * leaves at exactly its step count. A request larger than the whole budget is
+ * rejected (MCB_ERR_CAPACITY in its result). results[i] belongs to reqs[i];
+ * requests must be sorted by arrival time. */
+mcb_status mcb_serve_run(const mcb_serve_config* cfg, const mcb_request* reqs, int n,
+                         mcb_request_result* results, mcb_serve_stats* stats);
+/* A trace of n "serve" presets (seeds seed*1000003 + i) with Poisson arrivals
+ * at rate_per_s (exponential gaps from a second splitmix64 stream) and step
+ * counts multiplied by step_scale (at least 1 step). */
+mcb_status mcb_serve_trace(uint64_t seed, int n, double rate_per_s, double step_scale, mcb_request* out);
 
 /* The initial phi of mcb_init_bubbles, on the host (nx*ny*nz floats). */
 mcb_status mcb_host_init_phi(const mcb_params* p, int n, const mcb_bubble* bubbles, float width,

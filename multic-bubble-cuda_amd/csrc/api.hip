@@ -1,27 +1,30 @@
-// Context API of libmcb (include/mcb.h): device memory, initial state, the
-// time step on one HIP stream, readout and diagnostics.
+// Context and batch API of libmcb (include/mcb.h): device memory, initial
+// state, the time step on one HIP stream, readout and diagnostics.
 //
-// Two step paths share the same per-cell arithmetic (lattice.hpp):
-//   FUSED  (PERIODIC, FP32): k_phase + k_collide per step, pull storage
-//          with A/B population buffers (fused.hip).
-//   STAGED (both modes): the reference's kernel sequence of src/main.cu:79-178
-//          on the reference's buffers (ops.hip), one stream, no host syncs.
+// A context runs one of three engines, all sharing the per-cell arithmetic of
+// lattice.hpp:
+//   FUSED FP32 : k_phase + k_collide per step, pull storage, A/B buffers
+//                (fused.hip). Bit-identical to the FP32 transcription.
+//   16-bit     : FP16/BF16 population storage; the case runs as a batch of
+//                one (batch.hip, fused16.hip).
+//   STAGED     : the reference's kernel sequence of src/main.cu:79-178 on the
+//                reference's buffers (ops.hip), one stream, no host syncs.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "fused.hpp"
 
 using namespace mcb;
 
-// Host-side parameter helpers (host.cpp).
 namespace mcb {
 mcb_status host_validate(const mcb_params* p, std::string* why);
-Phys host_phys(const mcb_params& p);
 }
 
 struct mcb_ctx {
@@ -35,11 +38,11 @@ struct mcb_ctx {
     bool own_stream = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> tev;   // per-kernel timing events
-    Tiling tiling{64, 4, 16};
+    Tiling tiling{64, 8, 16};
     std::string err;
     std::vector<void*> allocs;
 
-    // FUSED state
+    // FUSED FP32 state
     float* f[2] = {nullptr, nullptr};
     float* g[2] = {nullptr, nullptr};
     int cur = 0;
@@ -54,7 +57,20 @@ struct mcb_ctx {
     float *pxx = nullptr, *pyy = nullptr, *pzz = nullptr, *pxy = nullptr, *pxz = nullptr, *pyz = nullptr;
     double* partial = nullptr;
     double* dsum = nullptr;
+    // 16-bit storage: a batch of one case
+    std::unique_ptr<Batch> batch;
+    int case_id = 0;
     bool initialised = false;
+};
+
+struct mcb_batch {
+    std::unique_ptr<Batch> b;
+    int device = 0;
+    int storage = MCB_STORE_FP32;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
 };
 
 namespace {
@@ -80,18 +96,22 @@ mcb_status hip_fail(mcb_ctx* c, hipError_t e, const char* where) {
 long long plane_stride(long long n) { return (n + 255) / 256 * 256; }
 
 // The fused kernels address all 19 planes of a buffer through one 32-bit
-// buffer resource (lattices up to ~56.5M cells).
+// buffer resource (FP32 lattices up to ~56.5M cells, 16-bit up to ~113M).
 bool fused_fits(const mcb_params& p) {
-    return plane_stride((long long)p.nx * p.ny * p.nz) <= kMaxFusedPlane;
+    const long long P = plane_stride((long long)p.nx * p.ny * p.nz);
+    return P * 19 * (p.storage == MCB_STORE_FP32 ? 4 : 2) <= 0xFFFFFFFFLL;
 }
 
 int resolve_path(const mcb_params& p) {
     if (p.path == MCB_PATH_STAGED) return MCB_PATH_STAGED;
     if (p.path == MCB_PATH_FUSED) return MCB_PATH_FUSED;
+    if (p.storage != MCB_STORE_FP32) return MCB_PATH_FUSED;
     return p.bc_mode == MCB_BC_PERIODIC && fused_fits(p) ? MCB_PATH_FUSED : MCB_PATH_STAGED;
 }
 
 size_t footprint(const mcb_params& p, int path) {
+    // 16-bit: the case plus the batch's ten macro fields.
+    if (p.storage != MCB_STORE_FP32) return case_bytes(p) + 10 * (size_t)plane_stride((long long)p.nx * p.ny * p.nz) * 4;
     const long long n = (long long)p.nx * p.ny * p.nz;
     const long long P = plane_stride(n);
     size_t floats = 0;
@@ -109,32 +129,28 @@ mcb_status alloc(mcb_ctx* c, float** out, size_t floats) {
     return MCB_OK;
 }
 
-CollideArgs collide_args(mcb_ctx* c) {
-    CollideArgs a{};
-    a.geo = c->geo;
-    a.phys = c->phys;
-    a.phi = c->phi;
-    a.fsrc = c->f[c->cur];
-    a.gsrc = c->g[c->cur];
-    a.fdst = c->f[c->cur ^ 1];
-    a.gdst = c->g[c->cur ^ 1];
-    a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
-    a.fx = c->ffx; a.fy = c->ffy; a.fz = c->ffz;
-    a.nx = c->normx; a.ny = c->normy; a.nz = c->normz;
-    a.zc = c->tiling.zc;
-    return a;
+CaseView fused_view(mcb_ctx* c) {
+    CaseView v{};
+    v.geo = c->geo;
+    v.phys = c->phys;
+    v.zc = c->tiling.zc;
+    v.fsrc = c->f[c->cur];
+    v.fdst = c->f[c->cur ^ 1];
+    v.gsrc = c->g[c->cur];
+    v.gdst = c->g[c->cur ^ 1];
+    v.phi = c->phi;
+    v.rho = c->rho; v.ux = c->ux; v.uy = c->uy; v.uz = c->uz;
+    v.fx = c->ffx; v.fy = c->ffy; v.fz = c->ffz;
+    v.nx = c->normx; v.ny = c->normy; v.nz = c->normz;
+    return v;
 }
 
 int step_mode(const mcb_ctx* c) { return c->p.tau_g == 1.0f ? COLLIDE_STEP : COLLIDE_STEP_BGK; }
 
-// One iteration of the fused path, optionally bracketing each pass with
-// events (ev[0..3]) for per-kernel timing.
-mcb_status fused_step(mcb_ctx* c, bool timed) {
-    if (timed) MCB_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    MCB_HIP(c, launch_phase(c->g[c->cur], c->phi, c->geo, c->stream));
-    if (timed) MCB_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    MCB_HIP(c, launch_collide(collide_args(c), step_mode(c), c->tiling, c->stream));
-    if (timed) MCB_HIP(c, hipEventRecord(c->ev[2], c->stream));
+mcb_status fused_step(mcb_ctx* c) {
+    const CaseView v = fused_view(c);
+    MCB_HIP(c, launch_phase(v, c->stream));
+    MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
     c->cur ^= 1;
     return MCB_OK;
 }
@@ -178,10 +194,24 @@ mcb_status staged_step(mcb_ctx* c) {
     return MCB_OK;
 }
 
+mcb_status batch_status(mcb_ctx* c, mcb_status st) {
+    if (st != MCB_OK) c->err = c->batch->error();
+    return st;
+}
+
 mcb_status check_ready(mcb_ctx* c) {
     if (!c) return MCB_ERR_INVALID_ARG;
     if (!c->initialised) return fail(c, MCB_ERR_INVALID_ARG, "context has no initial state (call mcb_init_*)");
     MCB_HIP(c, hipSetDevice(c->device));
+    return MCB_OK;
+}
+
+mcb_status any_step(mcb_ctx* c, int nsteps) {
+    if (c->batch) return batch_status(c, c->batch->step(nsteps));
+    for (int t = 0; t < nsteps; ++t) {
+        mcb_status st = c->path == MCB_PATH_FUSED ? fused_step(c) : staged_step(c);
+        if (st != MCB_OK) return st;
+    }
     return MCB_OK;
 }
 
@@ -242,7 +272,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
     if (st != MCB_OK) return st;
     const int path = resolve_path(*p);
     if (path == MCB_PATH_FUSED && (p->bc_mode != MCB_BC_PERIODIC || !fused_fits(*p))) return MCB_ERR_UNSUPPORTED;
-    if (p->storage != MCB_STORE_FP32) return MCB_ERR_UNSUPPORTED;
+    if (p->storage != MCB_STORE_FP32 && path != MCB_PATH_FUSED) return MCB_ERR_UNSUPPORTED;
 
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return MCB_ERR_NO_DEVICE;
@@ -272,7 +302,12 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
 
     const long long n = c->geo.n;
     const long long P = plane_stride(n);
-    if (path == MCB_PATH_FUSED) {
+    if (p->storage != MCB_STORE_FP32) {
+        c->geo.plane = P;
+        c->batch.reset(new (std::nothrow) Batch(c->device, c->stream, true));
+        if (!c->batch) return bail(MCB_ERR_OUT_OF_MEMORY);
+        c->tiling = choose_tiling(*p);
+    } else if (path == MCB_PATH_FUSED) {
         c->geo.plane = P;
         for (int b = 0; b < 2; ++b) {
             if ((st = alloc(c, &c->f[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
@@ -282,8 +317,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
                           &c->normx, &c->normy, &c->normz};
         for (auto* q : scal)
             if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
-        // Default tiling: widest x tile the lattice fills.
-        c->tiling = p->nx >= 64 ? Tiling{64, 8, 16} : (p->nx >= 32 ? Tiling{32, 8, 16} : Tiling{16, 16, 16});
+        c->tiling = choose_tiling(*p);
     } else {
         // The reference layout: planes of exactly n cells (src/header.cuh:21).
         c->geo.plane = n;
@@ -309,6 +343,7 @@ mcb_status mcb_destroy(mcb_ctx* c) {
     if (!c) return MCB_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->batch.reset();
     for (void* q : c->allocs) (void)hipFree(q);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -326,9 +361,57 @@ mcb_status mcb_init_bubbles(mcb_ctx* c, int n, const mcb_bubble* bubbles, float 
     return mcb_init_phi(c, phi.data(), phi.size());
 }
 
+extern "C++" {
+namespace mcb {
+// Initial state from bubbles computed on the device (periodic fused or
+// 16-bit contexts; used by the serving loop, where a host-side phi of a large
+// lattice would cost more than its steps). Not part of the C ABI.
+mcb_status ctx_init_bubbles_device(mcb_ctx* c, const BubbleSet& s) {
+    if (!c) return MCB_ERR_INVALID_ARG;
+    MCB_HIP(c, hipSetDevice(c->device));
+    if (c->batch) {
+        if (c->case_id) {
+            mcb_status st = batch_status(c, c->batch->remove(c->case_id));
+            if (st != MCB_OK) return st;
+            c->case_id = 0;
+        }
+        mcb_status st = batch_status(c, c->batch->add_bubbles(c->p, s.b, s.n, s.width, &c->case_id));
+        if (st != MCB_OK) return st;
+        if ((st = batch_status(c, c->batch->set_tiling(c->case_id, c->tiling))) != MCB_OK) return st;
+        c->initialised = true;
+        return MCB_OK;
+    }
+    if (c->path != MCB_PATH_FUSED || c->p.bc_mode != MCB_BC_PERIODIC)
+        return fail(c, MCB_ERR_UNSUPPORTED, "device initialisation needs the periodic fused path");
+    MCB_HIP(c, launch_bubble_phi(c->phi, c->geo, s, c->stream));
+    c->cur = 0;
+    const size_t pbytes = (size_t)c->geo.plane * sizeof(float);
+    float* zero[] = {c->rho, c->ux, c->uy, c->uz, c->ffx, c->ffy, c->ffz, c->normx, c->normy, c->normz};
+    for (float* q : zero) MCB_HIP(c, hipMemsetAsync(q, 0, pbytes, c->stream));
+    MCB_HIP(c, launch_init_pull(c->phi, c->f[0], c->g[0], c->geo, c->phys, c->stream));
+    MCB_HIP(c, hipStreamSynchronize(c->stream));
+    c->initialised = true;
+    return MCB_OK;
+}
+}  // namespace mcb
+}  // extern "C++"
+
 mcb_status mcb_init_phi(mcb_ctx* c, const float* phi_host, size_t n) {
     if (!c || !phi_host || n != (size_t)c->geo.n) return fail(c, MCB_ERR_INVALID_ARG, "phi size mismatch");
     MCB_HIP(c, hipSetDevice(c->device));
+    if (c->batch) {
+        if (c->case_id) {
+            mcb_status st = batch_status(c, c->batch->remove(c->case_id));
+            if (st != MCB_OK) return st;
+            c->case_id = 0;
+        }
+        mcb_status st = batch_status(c, c->batch->add(c->p, phi_host, &c->case_id));
+        if (st != MCB_OK) return st;
+        Tiling t = c->tiling;
+        if ((st = batch_status(c, c->batch->set_tiling(c->case_id, t))) != MCB_OK) return st;
+        c->initialised = true;
+        return MCB_OK;
+    }
     const size_t bytes = (size_t)c->geo.n * sizeof(float);
     const int NX = c->p.nx, NY = c->p.ny, NZ = c->p.nz;
     MCB_HIP(c, hipMemcpyAsync(c->phi, phi_host, bytes, hipMemcpyHostToDevice, c->stream));
@@ -356,11 +439,7 @@ mcb_status mcb_step(mcb_ctx* c, int nsteps) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
     if (nsteps < 0) return fail(c, MCB_ERR_INVALID_ARG, "negative step count");
-    for (int t = 0; t < nsteps; ++t) {
-        st = c->path == MCB_PATH_FUSED ? fused_step(c, false) : staged_step(c);
-        if (st != MCB_OK) return st;
-    }
-    return MCB_OK;
+    return any_step(c, nsteps);
 }
 
 mcb_status mcb_step_timed(mcb_ctx* c, int nsteps, double* ms) {
@@ -368,10 +447,7 @@ mcb_status mcb_step_timed(mcb_ctx* c, int nsteps, double* ms) {
     if (st != MCB_OK) return st;
     if (nsteps < 0 || !ms) return fail(c, MCB_ERR_INVALID_ARG, "bad arguments");
     MCB_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    for (int t = 0; t < nsteps; ++t) {
-        st = c->path == MCB_PATH_FUSED ? fused_step(c, false) : staged_step(c);
-        if (st != MCB_OK) return st;
-    }
+    if ((st = any_step(c, nsteps)) != MCB_OK) return st;
     MCB_HIP(c, hipEventRecord(c->ev[3], c->stream));
     MCB_HIP(c, hipEventSynchronize(c->ev[3]));
     float t = 0.0f;
@@ -383,9 +459,11 @@ mcb_status mcb_step_timed(mcb_ctx* c, int nsteps, double* ms) {
 mcb_status mcb_compute_macros(mcb_ctx* c) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
+    if (c->batch) return batch_status(c, c->batch->macros(c->case_id));
     if (c->path == MCB_PATH_FUSED) {
-        MCB_HIP(c, launch_phase(c->g[c->cur], c->phi, c->geo, c->stream));
-        MCB_HIP(c, launch_collide(collide_args(c), COLLIDE_MACROS, c->tiling, c->stream));
+        const CaseView v = fused_view(c);
+        MCB_HIP(c, launch_phase(v, c->stream));
+        MCB_HIP(c, launch_collide(v, COLLIDE_MACROS, c->tiling, c->stream));
         return MCB_OK;
     }
     return staged_macros(c);
@@ -395,6 +473,7 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
     if (!host) return fail(c, MCB_ERR_INVALID_ARG, "null host buffer");
+    if (c->batch) return batch_status(c, c->batch->read(c->case_id, field, host, n));
     const size_t cells = (size_t)c->geo.n;
     if (field == MCB_FIELD_F || field == MCB_FIELD_G) {
         if (n != 19 * cells) return fail(c, MCB_ERR_INVALID_ARG, "population buffer must hold 19*nx*ny*nz floats");
@@ -424,6 +503,7 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
     if (!host) return fail(c, MCB_ERR_INVALID_ARG, "null host buffer");
+    if (c->batch) return batch_status(c, c->batch->write(c->case_id, field, host, n));
     const size_t cells = (size_t)c->geo.n;
     if (field == MCB_FIELD_F || field == MCB_FIELD_G) {
         if (n != 19 * cells) return fail(c, MCB_ERR_INVALID_ARG, "population buffer must hold 19*nx*ny*nz floats");
@@ -451,6 +531,7 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
 mcb_status mcb_reduce(mcb_ctx* c, double* sum_phi, double* kinetic) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
+    if (c->batch) return batch_status(c, c->batch->reduce(c->case_id, sum_phi, kinetic));
     MCB_HIP(c, launch_reduce(c->phi, c->rho, c->ux, c->uy, c->uz, c->geo.n, c->partial, c->dsum, c->stream));
     double h[2] = {0.0, 0.0};
     MCB_HIP(c, hipMemcpyAsync(h, c->dsum, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -470,12 +551,13 @@ mcb_status mcb_synchronize(mcb_ctx* c) {
 
 // ---- per-kernel timing and tiling control (used by bench.py and the tuner)
 
-// Device time of each pass summed over nsteps fused steps:
+// Device time of each pass summed over nsteps fused FP32 steps:
 // out_ms[0] = k_phase, out_ms[1] = k_collide.
 mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_out) {
     mcb_status st = check_ready(c);
     if (st != MCB_OK) return st;
-    if (c->path != MCB_PATH_FUSED) return fail(c, MCB_ERR_UNSUPPORTED, "kernel times need the fused path");
+    if (c->path != MCB_PATH_FUSED || c->batch)
+        return fail(c, MCB_ERR_UNSUPPORTED, "kernel times need the fused FP32 path");
     if (!out_ms || n_out < 2 || nsteps < 1) return fail(c, MCB_ERR_INVALID_ARG, "bad arguments");
     // Three events per step, recorded back to back on the stream and read
     // once at the end, so the timed steps run without host round trips.
@@ -487,10 +569,11 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
     }
     for (int t = 0; t < nsteps; ++t) {
         hipEvent_t* e = &c->tev[3 * (size_t)t];
+        const CaseView v = fused_view(c);
         MCB_HIP(c, hipEventRecord(e[0], c->stream));
-        MCB_HIP(c, launch_phase(c->g[c->cur], c->phi, c->geo, c->stream));
+        MCB_HIP(c, launch_phase(v, c->stream));
         MCB_HIP(c, hipEventRecord(e[1], c->stream));
-        MCB_HIP(c, launch_collide(collide_args(c), step_mode(c), c->tiling, c->stream));
+        MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[2], c->stream));
         c->cur ^= 1;
     }
@@ -512,7 +595,12 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
 mcb_status mcb_set_tiling(mcb_ctx* c, int tx, int ty, int zc) {
     if (!c) return MCB_ERR_INVALID_ARG;
     Tiling t{tx, ty, zc};
-    if (!tiling_supported(t) || zc > c->p.nz) return fail(c, MCB_ERR_INVALID_ARG, "unsupported tiling");
+    const bool ok = c->p.storage == MCB_STORE_FP32 ? tiling_supported(t) : tiling_supported16(t);
+    if (!ok || zc > c->p.nz) return fail(c, MCB_ERR_INVALID_ARG, "unsupported tiling");
+    if (c->batch && c->case_id) {
+        mcb_status st = batch_status(c, c->batch->set_tiling(c->case_id, t));
+        if (st != MCB_OK) return st;
+    }
     c->tiling = t;
     return MCB_OK;
 }
@@ -534,6 +622,169 @@ mcb_status mcb_tiling_at(int i, int* tx, int* ty) {
     *ty = t.ty;
     return MCB_OK;
 }
+
+mcb_status mcb_autotune(const mcb_params* p, int* tx, int* ty, int* zc, double* ms_per_step) {
+    if (!p || host_validate(p, nullptr) != MCB_OK) return MCB_ERR_INVALID_ARG;
+    if (!fused_fits(*p)) return MCB_ERR_UNSUPPORTED;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= p->device) return MCB_ERR_NO_DEVICE;
+    Tiling t{};
+    double ms = 0.0;
+    mcb_status st = autotune(*p, p->device, &t, &ms);
+    if (st != MCB_OK) return st;
+    if (tx) *tx = t.tx;
+    if (ty) *ty = t.ty;
+    if (zc) *zc = t.zc;
+    if (ms_per_step) *ms_per_step = ms;
+    return MCB_OK;
+}
+
+// ---- batches
+
+size_t mcb_case_bytes(const mcb_params* p) {
+    if (!p || host_validate(p, nullptr) != MCB_OK) return 0;
+    return case_bytes(*p);
+}
+
+mcb_status mcb_batch_create(int device, int storage, void* stream, mcb_batch** out) {
+    if (!out) return MCB_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (storage < MCB_STORE_FP32 || storage > MCB_STORE_BF16) return MCB_ERR_INVALID_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count)
+        return MCB_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MCB_ERR_NO_DEVICE;
+    mcb_batch* b = new (std::nothrow) mcb_batch();
+    if (!b) return MCB_ERR_OUT_OF_MEMORY;
+    b->device = device;
+    b->storage = storage;
+    if (stream) {
+        b->stream = static_cast<hipStream_t>(stream);
+    } else if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess) {
+        b->own_stream = true;
+    } else {
+        delete b;
+        return MCB_ERR_HIP;
+    }
+    if (hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+        mcb_batch_destroy(b);
+        return MCB_ERR_HIP;
+    }
+    b->b.reset(new (std::nothrow) Batch(device, b->stream, true));
+    if (!b->b) {
+        mcb_batch_destroy(b);
+        return MCB_ERR_OUT_OF_MEMORY;
+    }
+    *out = b;
+    return MCB_OK;
+}
+
+mcb_status mcb_batch_destroy(mcb_batch* b) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    (void)hipSetDevice(b->device);
+    b->b.reset();
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return MCB_OK;
+}
+
+static mcb_status bstat(mcb_batch* b, mcb_status st) {
+    if (st != MCB_OK) b->err = b->b->error();
+    return st;
+}
+
+mcb_status mcb_batch_add_phi(mcb_batch* b, const mcb_params* p, const float* phi, size_t n, int* case_id) {
+    if (!b || !p || !phi || !case_id) return MCB_ERR_INVALID_ARG;
+    if (host_validate(p, nullptr) != MCB_OK) return MCB_ERR_INVALID_ARG;
+    if (n != (size_t)p->nx * p->ny * p->nz) return MCB_ERR_INVALID_ARG;
+    if (p->storage != b->storage || p->bc_mode != MCB_BC_PERIODIC) return MCB_ERR_UNSUPPORTED;
+    return bstat(b, b->b->add(*p, phi, case_id));
+}
+
+mcb_status mcb_batch_add(mcb_batch* b, const mcb_params* p, int n, const mcb_bubble* bubbles, float width,
+                         int* case_id) {
+    if (!b || !p || !case_id) return MCB_ERR_INVALID_ARG;
+    if (host_validate(p, nullptr) != MCB_OK) return MCB_ERR_INVALID_ARG;
+    std::vector<float> phi((size_t)p->nx * p->ny * p->nz);
+    mcb_status st = mcb_host_init_phi(p, n, bubbles, width, phi.data(), phi.size());
+    if (st != MCB_OK) return st;
+    return mcb_batch_add_phi(b, p, phi.data(), phi.size(), case_id);
+}
+
+mcb_status mcb_batch_remove(mcb_batch* b, int case_id) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->remove(case_id));
+}
+
+mcb_status mcb_batch_step(mcb_batch* b, int nsteps) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->step(nsteps));
+}
+
+mcb_status mcb_batch_step_timed(mcb_batch* b, int nsteps, double* ms) {
+    if (!b || !ms) return MCB_ERR_INVALID_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return MCB_ERR_NO_DEVICE;
+    if (b->b->count() > 0) {
+        // Build views, tables and graphs outside the timed region.
+        mcb_status st = bstat(b, b->b->step(0));
+        if (st != MCB_OK) return st;
+    }
+    if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) return MCB_ERR_HIP;
+    mcb_status st = bstat(b, b->b->step(nsteps));
+    if (st != MCB_OK) return st;
+    float t = 0.0f;
+    if (hipEventRecord(b->ev[1], b->stream) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&t, b->ev[0], b->ev[1]) != hipSuccess)
+        return MCB_ERR_HIP;
+    *ms = t;
+    return MCB_OK;
+}
+
+mcb_status mcb_batch_compute_macros(mcb_batch* b, int case_id) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->macros(case_id));
+}
+
+mcb_status mcb_batch_read_field(mcb_batch* b, int case_id, int field, float* host, size_t n) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->read(case_id, field, host, n));
+}
+
+mcb_status mcb_batch_write_field(mcb_batch* b, int case_id, int field, const float* host, size_t n) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->write(case_id, field, host, n));
+}
+
+mcb_status mcb_batch_reduce(mcb_batch* b, int case_id, double* sum_phi, double* kinetic) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->reduce(case_id, sum_phi, kinetic));
+}
+
+mcb_status mcb_batch_case_steps(mcb_batch* b, int case_id, long long* steps) {
+    if (!b || !steps) return MCB_ERR_INVALID_ARG;
+    CaseState* c = b->b->get(case_id);
+    if (!c) return MCB_ERR_INVALID_ARG;
+    *steps = c->steps;
+    return MCB_OK;
+}
+
+mcb_status mcb_batch_set_tiling(mcb_batch* b, int case_id, int tx, int ty, int zc) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->set_tiling(case_id, Tiling{tx, ty, zc}));
+}
+
+mcb_status mcb_batch_synchronize(mcb_batch* b) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->synchronize());
+}
+
+int mcb_batch_count(const mcb_batch* b) { return b ? b->b->count() : 0; }
+
+size_t mcb_batch_bytes(const mcb_batch* b) { return b ? b->b->bytes() : 0; }
+
+const char* mcb_batch_last_error(const mcb_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
 // ---- device helpers
 

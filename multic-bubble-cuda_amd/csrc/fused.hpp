@@ -1,4 +1,4 @@
-// Fused two-pass step of libmcb (PERIODIC mode), internal interface.
+// Fused two-pass periodic step of libmcb, internal interface.
 //
 // Storage ("pull" form): after a step, buffer f[cur] holds the
 // post-collision populations f*_l(x) at their source cell x, unstreamed, and
@@ -8,11 +8,11 @@
 // (collisionFluid pushes f*_l(x) to x + c_l, src/lbm.cu:285-286;
 // streamingCalc pulls g_l(x - c_l), src/lbm.cu:350-355).
 //
-// Pass 1, k_phase   : phi(x) = sum_l g*_l(x - c_l)              (phiCalc)
-// Pass 2, k_collide : gradCalc + curvatureCalc on LDS-staged phi and n,
-//                     momentiCalc on gathered f, collisionFluid and
-//                     collisionPhase, aligned stores of f*, g* to the other
-//                     buffer.
+// Pass 1, phase   : phi(x) = sum_l g*_l(x - c_l)              (phiCalc)
+// Pass 2, collide : gradCalc + curvatureCalc on LDS-staged phi and n,
+//                   momentiCalc on gathered f, collisionFluid and
+//                   collisionPhase, aligned stores of f*, g* to the other
+//                   buffer.
 #pragma once
 #include "common.hpp"
 
@@ -24,14 +24,35 @@ struct Tiling {
     int tx, ty, zc;   // x/y tile of a workgroup, z cells marched per workgroup
 };
 
-struct CollideArgs {
+// Mass bookkeeping of a case with 16-bit populations (device memory). The
+// phase pass adds delta[q] to every phi it computes at case parity q, writes
+// one FP64 partial sum of phi per work item, and the last work item to
+// finish sums the partials in a fixed order and stores
+// delta[q ^ 1] = (target - sum phi) / cells for the next step. The counter
+// returns to 0 after every pass.
+struct MassCell {
+    double target;          // sum of phi the case must keep
+    double delta[2];
+    unsigned counter;
+    unsigned reserved;
+};
+
+// Everything a kernel needs about one case (one lattice).
+struct CaseView {
     Geo geo;
     Phys phys;
-    const float* phi;
-    const float* fsrc;
-    const float* gsrc;
-    float* fdst;
-    float* gdst;
+    int zc;                 // z planes per collide workgroup
+    int mslot;              // parity of the case's state: delta slot the phase pass reads
+    MassCell* mass;         // 16-bit storage only
+    double* partial;        // one per work item of the case's tiling
+    const void* fsrc;       // populations of the previous step (pull form)
+    void* fdst;             // populations of this step
+    const void* gsrc;
+    void* gdst;
+    float* phi;             // phi of this step: written by the phase pass
+    const float* phi_prev;  // 16-bit storage: phi of the previous step (anchor of g)
+    const float* rho_prev;  // 16-bit storage: rho of the previous step (anchor of f)
+    float* rho_next;        // 16-bit storage: rho of this step
     // COLLIDE_MACROS outputs
     float* rho;
     float* ux;
@@ -43,21 +64,64 @@ struct CollideArgs {
     float* nx;
     float* ny;
     float* nz;
-    int zc;
+};
+
+// One work item of a batched launch: case index, tile origin x0, y0, z0.
+struct WorkItem {
+    int c, x0, y0, z0;
 };
 
 bool tiling_supported(const Tiling& t);
 int tiling_count();
 Tiling tiling_at(int i);
 
-hipError_t launch_phase(const float* g, float* phi, const Geo& geo, hipStream_t s);
-hipError_t launch_collide(const CollideArgs& a, int mode, const Tiling& t, hipStream_t s);
+// ---- single case, FP32 populations (bit-exact form)
+hipError_t launch_phase(const CaseView& c, hipStream_t s);
+hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStream_t s);
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
 // initTensor + initDist (src/init.cu:21-24, 69-79).
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->
 // reference layout; sign = -1: reference layout -> pull form).
 hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int sign, hipStream_t s);
+
+// ---- batched (many cases, one launch per pass), any storage form
+// storage: MCB_STORE_FP32 / FP16 / BF16; work: device array of n_work items.
+hipError_t launch_phase_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
+                              int storage, hipStream_t s);
+hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
+                                int mode, int storage, hipStream_t s);
+
+// ---- 16-bit populations (a single case runs as a batch of one)
+bool tiling_supported16(const Tiling& t);
+// 16-bit initial state: g*_l(y) = w_l phi(y + c_l) against the anchor phi(y),
+// f*_l = w_l against the anchor rho = 1 (offset 0).
+hipError_t launch_init16(const float* phi, void* f, void* g, float* rho_anchor, const Geo& geo, const Phys& p,
+                         int storage, hipStream_t s);
+// Decode 16-bit pull-form populations to FP32 reference layout:
+// dst_l(x) = w_l anchor(x - c_l) + t_l(x - c_l) / K_l.
+hipError_t launch_decode16(const void* src, const float* anchor, float* dst, const Geo& geo, const Phys& p,
+                           int storage, hipStream_t s);
+// Encode FP32 reference-layout populations to 16-bit pull form against the
+// anchor field: t_l(x) = (src_l(x + c_l) - w_l anchor(x)) K_l.
+hipError_t launch_encode16(const float* src, const float* anchor, void* dst, const Geo& geo, const Phys& p,
+                           int storage, hipStream_t s);
+// field[i] += delta for i < n (uniform phase correction of the drift).
+hipError_t launch_add_scalar(float* field, long long n, const double* delta, hipStream_t s);
+
+// Bubbles of a periodic initial state, passed by value to the init kernel.
+constexpr int kMaxBubbleSet = 8;
+struct BubbleSet {
+    mcb_bubble b[kMaxBubbleSet];
+    int n;
+    float width;
+};
+// Periodic initial phi of a bubble set on the device (the host formula,
+// device tanhf/sqrtf).
+hipError_t launch_bubble_phi(float* phi, const Geo& geo, const BubbleSet& bubbles, hipStream_t s);
+// *out = sum of v[0..n) in FP64 (partial: kReduceBlocks doubles of scratch).
+hipError_t launch_sum(const float* v, long long n, double* partial, double* out, hipStream_t s);
+
 // dst = src for bytes (a multiple of 16) with a plain float4 copy kernel.
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // Deterministic FP64 sums: out[0] = sum phi, out[1] = sum 0.5*rho*|u|^2.

@@ -1,6 +1,7 @@
 // Host-side part of libmcb: parameter checks, physical constants, the
 // initial bubble field and the reference's output formats. No device calls,
 // so all of it runs (and is tested) without a GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -294,7 +295,46 @@ mcb_status mcb_case_preset(const char* name, uint64_t seed, mcb_case* out) {
         }
         return MCB_ERR_INVALID_ARG;
     }
+    if (nm == "batch" || nm == "serve") {
+        const bool serve = nm == "serve";
+        int side;
+        if (serve) {
+            side = 48 + 16 * (int)(10.0 * rng.uniform());          // 48, 64, ..., 192
+        } else {
+            static const int sides[5] = {32, 48, 64, 96, 128};
+            side = sides[(int)(5.0 * rng.uniform())];
+        }
+        const double rlo = serve ? 0.1 : 0.15;
+        const float r = (float)(side * (rlo + (0.3 - rlo) * rng.uniform()));
+        const float sigma = (float)(0.01 + 0.09 * rng.uniform());
+        const float tau_f = (float)(0.52 + 0.28 * rng.uniform());
+        int nsteps = 2000;
+        if (serve) nsteps = (int)std::lround(500.0 * std::pow(10.0, rng.uniform()));
+        periodic_case(out, side, tau_f, 1.0f, sigma, nsteps);
+        if (serve) out->params.storage = MCB_STORE_FP16;
+        out->n_bubbles = 1;
+        const float cx = (float)(side * rng.uniform()), cy = (float)(side * rng.uniform());
+        const float cz = (float)(side * rng.uniform());
+        out->bubbles[0] = mcb_bubble{cx, cy, cz, r};
+        return MCB_OK;
+    }
     return MCB_ERR_INVALID_ARG;
+}
+
+mcb_status mcb_serve_trace(uint64_t seed, int n, double rate_per_s, double step_scale, mcb_request* out) {
+    if (n < 0 || (n > 0 && !out) || !(rate_per_s > 0.0) || !(step_scale > 0.0)) return MCB_ERR_INVALID_ARG;
+    SplitMix64 gaps{seed ^ 0xA5A5A5A5DEADBEEFULL};
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) {
+        std::memset(&out[i], 0, sizeof(out[i]));
+        mcb_status st = mcb_case_preset("serve", seed * 1000003ULL + (uint64_t)i, &out[i].c);
+        if (st != MCB_OK) return st;
+        out[i].c.nsteps = std::max(1, (int)std::lround(out[i].c.nsteps * step_scale));
+        // Exponential gap: -ln(1 - U) / rate, U in [0, 1).
+        t += -std::log1p(-gaps.uniform()) / rate_per_s;
+        out[i].arrival_s = t;
+    }
+    return MCB_OK;
 }
 
 }  // extern "C"

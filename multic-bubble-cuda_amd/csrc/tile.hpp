@@ -1,0 +1,467 @@
+// Tile bodies of the two passes of a step, shared by the single-case and the
+// batched kernels and by every population storage form.
+//
+// Storage forms (template parameter S):
+//   StoreF32  : populations stored as FP32, read and written as computed.
+//               This form reproduces the FP32 transcription bit for bit.
+//   StoreF16 / StoreBF16 : populations stored in 16 bits as offsets from
+//               w_l times an FP32 anchor of their source cell:
+//                 f*_l(y) = w_l rho(y) + t_l(y) / K_l,  g*_l(y) = w_l phi(y) + t_l(y) / K_l,
+//               where rho(y), phi(y) are the density and phase the source cell
+//               had when it collided (kept as FP32 fields). The offsets are
+//               small (velocity and sharpening terms), so 16 bits keep about
+//               1e-5 of w_l. K_l is a power of two near 2^12 / w_l: an
+//               exact scale, so encoding and decoding carry no systematic
+//               bias (a rounded 1/w_l would scale every population by the
+//               same w_l * fl(1/w_l) != 1 and drift the mass each step), and
+//               large enough that offsets down to ~1e-8 w_l stay in FP16's
+//               normal range instead of losing bits as subnormals. FP16's
+//               largest value then bounds an offset to 12 w_0, 18 w_1, 18 w_2.
+//               All arithmetic stays FP32. Each cell's rounding residual of
+//               links 1..18 is folded into its rest link (l = 0), so the
+//               cell's total is stored with one 16-bit rounding only.
+#pragma once
+#include "fused.hpp"
+#include "lattice_fast.hpp"
+
+namespace mcb {
+
+// Offset scale K of the weight classes (rest, face, edge) and its inverse.
+__host__ __device__ constexpr float off_scale(int wc) {
+    return wc == 0 ? 16384.0f : (wc == 1 ? 65536.0f : 131072.0f);
+}
+__host__ __device__ constexpr float off_unscale(int wc) {
+    return wc == 0 ? 0x1p-14f : (wc == 1 ? 0x1p-16f : 0x1p-17f);
+}
+
+// Population of a 16-bit form from its anchor and its decoded offset t.
+template <int L> __device__ __forceinline__ float pop_from_offset(const Phys& p, float anchor, float t) {
+    return __builtin_fmaf(p.w[WC[L]], anchor, t * off_unscale(WC[L]));
+}
+// Offset of population v against its anchor (before rounding to 16 bits).
+template <int L> __device__ __forceinline__ float offset_of_pop(const Phys& p, float anchor, float v) {
+    return __builtin_fmaf(-p.w[WC[L]], anchor, v) * off_scale(WC[L]);
+}
+
+__device__ __forceinline__ int wrapmod(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+template <int C, class T> __device__ __forceinline__ T pick(T m, T z, T p) {
+    if constexpr (C < 0) return m;
+    else if constexpr (C > 0) return p;
+    else return z;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), and each XCD has its own L2. xcd_order renumbers them so every
+// XCD walks one contiguous range of tiles: x-neighbouring tiles, which share
+// the cache lines that the x-shifted gathers only partly use, then meet in one
+// L2 instead of fetching those lines from HBM twice. Bijective on [0, total).
+constexpr unsigned kXcds = 8;
+__host__ __device__ __forceinline__ unsigned xcd_order(unsigned lid, unsigned total) {
+    const unsigned q = total / kXcds, r = total % kXcds;
+    const unsigned xcd = lid % kXcds, slot = lid / kXcds;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
+
+// ------------------------------------------------------------ storage forms
+
+struct StoreF32 {
+    static constexpr bool kAnchored = false;
+    static constexpr unsigned kBytes = 4;
+};
+struct StoreF16 {
+    static constexpr bool kAnchored = true;
+    static constexpr unsigned kBytes = 2;
+    static __device__ __forceinline__ float dec(unsigned short b) { return (float)__builtin_bit_cast(_Float16, b); }
+    static __device__ __forceinline__ unsigned short enc(float v) {
+        return __builtin_bit_cast(unsigned short, (_Float16)v);
+    }
+};
+struct StoreBF16 {
+    static constexpr bool kAnchored = true;
+    static constexpr unsigned kBytes = 2;
+    static __device__ __forceinline__ float dec(unsigned short b) { return (float)__builtin_bit_cast(__bf16, b); }
+    static __device__ __forceinline__ unsigned short enc(float v) { return __builtin_bit_cast(unsigned short, (__bf16)v); }
+};
+
+// 19 population planes of one buffer behind one buffer resource (see Planes
+// in common.hpp); 16-bit forms move raw offsets.
+template <class S> struct PopPlanes;
+template <> struct PopPlanes<StoreF32> : Planes {
+    __device__ __forceinline__ PopPlanes(const void* base, unsigned plane_elems)
+        : Planes(static_cast<const float*>(base), plane_elems) {}
+};
+template <class S> struct PopPlanes {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned plane_bytes;
+    __device__ __forceinline__ PopPlanes(const void* base, unsigned plane_elems)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(19u * plane_elems * 2u),
+                                              kBufferConfigWord)),
+          plane_bytes(plane_elems * 2u) {}
+    template <int L> __device__ __forceinline__ float ld(unsigned elem) const {
+        return S::dec(__builtin_amdgcn_raw_buffer_load_b16(r, elem << 1, L * plane_bytes, 0));
+    }
+    // Stores the 16-bit form of v and returns the value it decodes to.
+    template <int L> __device__ __forceinline__ float st(unsigned elem, float v) const {
+        const unsigned short b = S::enc(v);
+        __builtin_amdgcn_raw_buffer_store_b16(b, r, elem << 1, L * plane_bytes, MCB_STORE_AUX);
+        return S::dec(b);
+    }
+};
+
+// ------------------------------------------------------------------ phase
+
+// phiCalc (src/lbm.cu:18-25) for one cell of a pull-form state: the
+// post-stream g_l(x) is g*_l(x - c_l). xs/ys/zs: offsets of x-1, x, x+1 etc.
+template <class S>
+__device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const unsigned (&xs)[3],
+                                           const unsigned (&ys)[3], const unsigned (&zs)[3]) {
+    float sum = 0.0f;
+    for_links([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
+        if constexpr (S::kAnchored) sum += pop_from_offset<l>(c.phys, ld32(c.phi_prev, src), gp.template ld<l>(src));
+        else sum += gp.template ld<l>(src);
+    });
+    return sum;
+}
+
+// ---------------------------------------------------------------- collide
+
+// One cell of the collide pass for a 16-bit form (see lattice_fast.hpp):
+// surface-tension force, moments, and the post-collision offsets of f and g
+// written against this cell's rho and phi. Links 1..18 go first; their
+// rounding residuals (in population units) then go into the rest link.
+template <int MODE, class S, class NBN>
+__device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k, const PopPlanes<S>& fsrc,
+                                       const PopPlanes<S>& gsrc, const PopPlanes<S>& fdst,
+                                       const PopPlanes<S>& gdst, unsigned me, const unsigned (&xs)[3],
+                                       const unsigned (&ys)[3], const unsigned (&zs)[3], NBN&& nbn, float ncx,
+                                       float ncy, float ncz, float ind, float phic) {
+    float Fx, Fy, Fz;
+    fast::curvature_force(nbn, k, ncx, ncy, ncz, ind, Fx, Fy, Fz);
+    float fv[Q];
+    for_links([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
+        fv[l] = pop_from_offset<l>(c.phys, ld32(c.rho_prev, src), fsrc.template ld<l>(src));
+    });
+    const fast::Fluid fl = fast::fluid(fv, k, Fx, Fy, Fz);
+    if constexpr (MODE == COLLIDE_MACROS) {
+        st32(c.rho, me, fl.rho);
+        st32(c.ux, me, fl.ux);
+        st32(c.uy, me, fl.uy);
+        st32(c.uz, me, fl.uz);
+        st32(c.fx, me, Fx);
+        st32(c.fy, me, Fy);
+        st32(c.fz, me, Fz);
+        st32(c.nx, me, ncx);
+        st32(c.ny, me, ncy);
+        st32(c.nz, me, ncz);
+    } else {
+        st32(c.rho_next, me, fl.rho);
+        const float phin = k.sharp_c * phic * (1.0f - phic);
+        // g offset of link l after the phase collision, from its odd part.
+        auto g_off = [&](auto lc, float odd) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (MODE == COLLIDE_STEP) {
+                return odd;
+            } else {
+                // (1 - omega_g)(g_pre - w phi) + omega_g (+-odd), with
+                // g_pre - w phi = w (anchor - phi) + t / K.
+                const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
+                const float pre = __builtin_fmaf(k.template w<l>(), ld32(c.phi_prev, src) - phic,
+                                                 gsrc.template ld<l>(src) * off_unscale(WC[l]));
+                return k.one_m_omega_g * pre + k.omega_g * odd;
+            }
+        };
+        float rf = 0.0f, rg = 0.0f;
+        auto put = [&](auto lc, const PopPlanes<S>& dst, float off, float& r) {
+            constexpr int l = decltype(lc)::value;
+            const float t = off * off_scale(WC[l]);
+            r += (t - dst.template st<l>(me, t)) * off_unscale(WC[l]);
+        };
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (l % 2 == 1) {
+                float op, om;
+                fast::fluid_pair<l>(fl, k, Fx, Fy, Fz, op, om);
+                put(LinkC<l>{}, fdst, op, rf);
+                put(LinkC<l + 1>{}, fdst, om, rf);
+                const float odd = fast::phase_odd<l>(k, phic, phin, fl.ux, fl.uy, fl.uz, ncx, ncy, ncz);
+                put(LinkC<l>{}, gdst, g_off(LinkC<l>{}, odd), rg);
+                put(LinkC<l + 1>{}, gdst, g_off(LinkC<l + 1>{}, -odd), rg);
+            }
+        });
+        fdst.template st<0>(me, (fast::fluid_rest(fl, k) + rf) * off_scale(0));
+        gdst.template st<0>(me, (g_off(LinkC<0>{}, 0.0f) + rg) * off_scale(0));
+    }
+}
+
+// gradCalc + curvatureCalc + momentiCalc + collisionFluid + collisionPhase
+// for the TX x TY column (x0, y0) of case c over planes [zb, ze). phi planes
+// (2-cell halo) live in a 4-slot LDS ring and the normals (1-cell halo) in a
+// 3-slot ring, so the gradient is computed once per tile-plus-halo cell and
+// each phi value is fetched about (1 + 4/zc) times. Must be called by all
+// TX*TY threads of the workgroup (it contains barriers).
+template <int TX, int TY, int MODE, class S>
+__device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, int zb, int ze) {
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 4, PY = TY + 4, PN = PX * PY;  // phi plane + 2-halo
+    constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;  // normal plane + 1-halo
+    constexpr int KP = (PN + NT - 1) / NT;
+    constexpr int KH = (HN + NT - 1) / NT;
+    __shared__ float s_phi[4][PN];
+    __shared__ float s_n[3][3][HN];
+    __shared__ float s_ind[2][NT];
+
+    const Geo& G = c.geo;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const Derived d(c.phys);
+    const fast::Consts k16(c.phys);
+    const PopPlanes<S> fsrc(c.fsrc, (unsigned)G.plane), fdst(c.fdst, (unsigned)G.plane);
+    const PopPlanes<S> gsrc(c.gsrc, (unsigned)G.plane), gdst(c.gdst, (unsigned)G.plane);
+
+    // In-plane offsets of the halo phi elements this thread stages.
+    unsigned poff[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        const int e = tid + q * NT;
+        unsigned off = 0;
+        if (e < PN) {
+            const int lx = e % PX, ly = e / PX;
+            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx));
+        }
+        poff[q] = off;
+    }
+
+    auto load_phi = [&](int z) {
+        const float* __restrict__ src = c.phi + (size_t)wrapmod(z, G.nz) * G.nxy;
+        float* dst = s_phi[z & 3];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = ld32(src, poff[q]);
+        }
+    };
+    auto nslot = [](int z) { return (z + 3) % 3; };  // z >= -3
+    auto comp_n = [&](int z) {
+        const float* pm = s_phi[(z - 1) & 3];
+        const float* p0 = s_phi[z & 3];
+        const float* pp = s_phi[(z + 1) & 3];
+        const int sl = nslot(z);
+        float* ind_dst = s_ind[z & 1];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN) {
+                const int hx = e % HX, hy = e / HX;
+                const int cc = (hy + 1) * PX + (hx + 1);
+                float nx, ny, nz, ind;
+                auto nb = [&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    return pick<CZ[l]>(pm, p0, pp)[cc + CY[l] * PX + CX[l]];
+                };
+                if constexpr (S::kAnchored) fast::grad_normal(nb, k16, nx, ny, nz, ind);
+                else grad_normal(nb, d, nx, ny, nz, ind);
+                s_n[sl][0][e] = nx;
+                s_n[sl][1][e] = ny;
+                s_n[sl][2][e] = nz;
+                if (hx >= 1 && hx <= TX && hy >= 1 && hy <= TY) ind_dst[(hy - 1) * TX + (hx - 1)] = ind;
+            }
+        }
+    };
+
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    // Unsigned 32-bit offsets: the gathers use a scalar base + 32-bit vector
+    // offset instead of 64-bit per-lane addresses.
+    const unsigned xs[3] = {valid ? (unsigned)shift<-1>(x, G.nx) : 0u, (unsigned)x,
+                            valid ? (unsigned)shift<1>(x, G.nx) : 0u};
+    const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
+                            valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
+
+    // Prologue: phi planes zb-2..zb+1, normals of planes zb-1 and zb.
+    load_phi(zb - 2);
+    load_phi(zb - 1);
+    load_phi(zb);
+    load_phi(zb + 1);
+    __syncthreads();
+    comp_n(zb - 1);
+    comp_n(zb);
+    __syncthreads();
+
+    for (int z = zb; z < ze; ++z) {
+        // Slot (z+2)&3 held plane z-2, last read by comp_n(z-1) before the
+        // previous barrier.
+        load_phi(z + 2);
+        __syncthreads();
+        // n slot of z+1 held plane z-2 (read last in iteration z-1).
+        comp_n(z + 1);
+        __syncthreads();
+        if (!valid) continue;
+
+        const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                                (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+        const unsigned me = zs[1] + ys[1] + xs[1];
+        const int sm = nslot(z - 1), s0 = nslot(z), sp = nslot(z + 1);
+        const int e0 = (ty + 1) * HX + (tx + 1);
+        const float ncx = s_n[s0][0][e0], ncy = s_n[s0][1][e0], ncz = s_n[s0][2][e0];
+        const float ind = s_ind[z & 1][tid];
+
+        const float phic = s_phi[z & 3][(ty + 2) * PX + (tx + 2)];
+        auto nbn = [&](auto lc, float& ax, float& ay, float& az) {
+            constexpr int l = decltype(lc)::value;
+            const int slot = pick<CZ[l]>(sm, s0, sp);
+            const int e = e0 + CY[l] * HX + CX[l];
+            if constexpr (CX[l] != 0) ax = s_n[slot][0][e];
+            if constexpr (CY[l] != 0) ay = s_n[slot][1][e];
+            if constexpr (CZ[l] != 0) az = s_n[slot][2][e];
+        };
+        if constexpr (S::kAnchored) {
+            cell16<MODE>(c, k16, fsrc, gsrc, fdst, gdst, me, xs, ys, zs, nbn, ncx, ncy, ncz, ind, phic);
+            continue;
+        }
+
+        float curv, Fx, Fy, Fz;
+        curvature_force(nbn, c.phys, d, ncx, ncy, ncz, ind, curv, Fx, Fy, Fz);
+
+        float fv[Q];
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            fv[l] = fsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+        });
+        const Moments m = moments(fv, c.phys, Fx, Fy, Fz);
+
+        if constexpr (MODE == COLLIDE_MACROS) {
+            st32(c.rho, me, m.rho);
+            st32(c.ux, me, m.ux);
+            st32(c.uy, me, m.uy);
+            st32(c.uz, me, m.uz);
+            st32(c.fx, me, Fx);
+            st32(c.fy, me, Fy);
+            st32(c.fz, me, Fz);
+            st32(c.nx, me, ncx);
+            st32(c.ny, me, ncy);
+            st32(c.nz, me, ncz);
+        } else {
+            const FluidCell fc = fluid_cell(m, Fx, Fy, Fz);
+            const PhaseCell pc = phase_cell(c.phys, phic, m.ux, m.uy, m.uz, ncx, ncy, ncz);
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                fdst.template st<l>(me, fluid_post<l>(fc, c.phys, d));
+            });
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (MODE == COLLIDE_STEP) {
+                    gdst.template st<l>(me, phase_post<l>(pc, c.phys));
+                } else {
+                    const float gpre = gsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+                    gdst.template st<l>(me, phase_post_bgk<l>(pc, c.phys, d, gpre));
+                }
+            });
+        }
+    }
+}
+
+// ---------------------------------------------------------------- batched
+
+// One work item per workgroup: a TX x TY x zc block of one case. The host
+// builds the work table case by case, x fastest, then renumbers it with
+// xcd_order so each XCD walks contiguous tiles.
+// Sum of v over the NT threads of the workgroup in a fixed order (butterfly
+// within each wave64, then the waves in order); the result is valid in
+// thread 0. Contains a barrier.
+template <int NT> __device__ __forceinline__ double block_sum(double v) {
+    static_assert(NT % 64 == 0, "whole waves");
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __shared__ double s_w[NT / 64];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if (tid % 64 == 0) s_w[tid / 64] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < NT / 64; ++i) s += s_w[i];
+    return s;
+}
+
+// Mass bookkeeping of one phase work item (see MassCell): publish the
+// partial, and if this is the case's last work item, turn all partials into
+// the next step's delta.
+template <int TX, int TY>
+__device__ __forceinline__ void mass_partial(const CaseView& c, const WorkItem& w, double part) {
+    constexpr int NT = TX * TY;
+    const Geo& G = c.geo;
+    const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
+    const unsigned n_tiles = gx * gy * (unsigned)((G.nz + c.zc - 1) / c.zc);
+    const unsigned tile = ((unsigned)(w.z0 / c.zc) * gy + (unsigned)(w.y0 / TY)) * gx + (unsigned)(w.x0 / TX);
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    __shared__ int s_last;
+    const double s = block_sum<NT>(part);
+    if (tid == 0) {
+        __hip_atomic_store(&c.partial[tile], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned done =
+            __hip_atomic_fetch_add(&c.mass->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = done == n_tiles - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double acc = 0.0;
+    for (unsigned i = (unsigned)tid; i < n_tiles; i += NT)
+        acc += __hip_atomic_load(&c.partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double total = block_sum<NT>(acc);
+    if (tid == 0) {
+        c.mass->delta[c.mslot ^ 1] = (c.mass->target - total) / (double)G.n;
+        __hip_atomic_store(&c.mass->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int TX, int TY, class S>
+__global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restrict__ cases,
+                                                         const WorkItem* __restrict__ work) {
+    const WorkItem w = work[blockIdx.x];
+    const CaseView c = cases[w.c];
+    const Geo& G = c.geo;
+    const int x = w.x0 + (int)threadIdx.x, y = w.y0 + (int)threadIdx.y;
+    const bool valid = x < G.nx && y < G.ny;
+    if constexpr (!S::kAnchored) {
+        if (!valid) return;
+    }
+    const unsigned xs[3] = {valid ? (unsigned)shift<-1>(x, G.nx) : 0u, (unsigned)x,
+                            valid ? (unsigned)shift<1>(x, G.nx) : 0u};
+    const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
+                            valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
+    const PopPlanes<S> gp(c.gsrc, (unsigned)G.plane);
+    const int ze = min(w.z0 + c.zc, G.nz);
+    double part = 0.0;
+    float delta = 0.0f;
+    if constexpr (S::kAnchored) delta = (float)c.mass->delta[c.mslot];
+    if (valid) {
+        for (int z = w.z0; z < ze; ++z) {
+            const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                                    (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+            float v = phase_sum<S>(c, gp, xs, ys, zs);
+            if constexpr (S::kAnchored) {
+                v += delta;
+                part += (double)v;
+            }
+            st32(c.phi, zs[1] + ys[1] + xs[1], v);
+        }
+    }
+    if constexpr (S::kAnchored) mass_partial<TX, TY>(c, w, part);
+}
+
+template <int TX, int TY, int MODE, class S>
+__global__ __launch_bounds__(TX * TY) void k_collide_batch(const CaseView* __restrict__ cases,
+                                                           const WorkItem* __restrict__ work) {
+    const WorkItem w = work[blockIdx.x];
+    const CaseView c = cases[w.c];
+    collide_tile<TX, TY, MODE, S>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, c.geo.nz));
+}
+
+}  // namespace mcb

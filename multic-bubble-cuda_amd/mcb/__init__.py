@@ -61,6 +61,38 @@ class Case(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("c", Case), ("arrival_s", ctypes.c_double), ("phi_out", ctypes.POINTER(ctypes.c_float))]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+        ("sum_phi", ctypes.c_double), ("kinetic", ctypes.c_double), ("lattice_updates", ctypes.c_longlong),
+    ]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int), ("storage", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+        ("max_cases", ctypes.c_int), ("tick_steps", ctypes.c_int), ("sequential", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("makespan_s", ctypes.c_double), ("lattice_updates", ctypes.c_double), ("mlups", ctypes.c_double),
+        ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double),
+        ("max_s", ctypes.c_double), ("completed", ctypes.c_int), ("rejected", ctypes.c_int),
+        ("peak_cases", ctypes.c_int), ("rounds", ctypes.c_int), ("peak_bytes", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class OpConsts(ctypes.Structure):
     _fields_ = [
         ("w", ctypes.c_float * 3), ("omega", ctypes.c_float), ("sharp_c", ctypes.c_float),
@@ -113,6 +145,28 @@ PROTOTYPES = {
     "mcb_device_synchronize": (_i, []),
     "mcb_op_consts_from_params": (_i, [_PP, _OCP]),
     "mcb_bench_copy": (_i, [_i, _sz, _i, _dp]),
+    "mcb_autotune": (_i, [_PP, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), _dp]),
+    "mcb_case_bytes": (_sz, [_PP]),
+    "mcb_batch_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
+    "mcb_batch_destroy": (_i, [_vp]),
+    "mcb_batch_add": (_i, [_vp, _PP, _i, ctypes.POINTER(Bubble), _f, ctypes.POINTER(_i)]),
+    "mcb_batch_add_phi": (_i, [_vp, _PP, _fp, _sz, ctypes.POINTER(_i)]),
+    "mcb_batch_remove": (_i, [_vp, _i]),
+    "mcb_batch_step": (_i, [_vp, _i]),
+    "mcb_batch_step_timed": (_i, [_vp, _i, _dp]),
+    "mcb_batch_compute_macros": (_i, [_vp, _i]),
+    "mcb_batch_read_field": (_i, [_vp, _i, _i, _fp, _sz]),
+    "mcb_batch_write_field": (_i, [_vp, _i, _i, _fp, _sz]),
+    "mcb_batch_reduce": (_i, [_vp, _i, _dp, _dp]),
+    "mcb_batch_case_steps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "mcb_batch_set_tiling": (_i, [_vp, _i, _i, _i, _i]),
+    "mcb_batch_synchronize": (_i, [_vp]),
+    "mcb_batch_count": (_i, [_vp]),
+    "mcb_batch_bytes": (_sz, [_vp]),
+    "mcb_batch_last_error": (_cp, [_vp]),
+    "mcb_serve_run": (_i, [ctypes.POINTER(ServeConfig), ctypes.POINTER(Request), _i, ctypes.POINTER(RequestResult),
+                           ctypes.POINTER(ServeStats)]),
+    "mcb_serve_trace": (_i, [ctypes.c_uint64, _i, _d, _d, ctypes.POINTER(Request)]),
 }
 # mcb_ops.h: reference kernel names -> number of device pointer arguments.
 OP_POINTERS = {
@@ -315,6 +369,140 @@ class Lattice:
         """phi, rho, ux, uy, uz of the current state (a17: the macros of the next iteration)."""
         self.compute_macros()
         return {k: self.read(k) for k in ("phi", "rho", "ux", "uy", "uz")}
+
+
+def autotune(params):
+    """Time every tile shape on a scratch lattice; returns (tx, ty, zc, ms_per_step)."""
+    a, b, c, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    check(lib().mcb_autotune(ctypes.byref(params), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                             ctypes.byref(ms)), what="mcb_autotune")
+    return a.value, b.value, c.value, ms.value
+
+
+def case_bytes(params):
+    return lib().mcb_case_bytes(ctypes.byref(params))
+
+
+class Batch:
+    """Many independent periodic cases advanced together (mcb_batch_*)."""
+
+    def __init__(self, storage=STORE_FP32, device=0, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().mcb_batch_create(device, storage, stream, ctypes.byref(h)), what="mcb_batch_create")
+        self._h = h
+        self.storage = storage
+        self.shapes = {}
+
+    def _chk(self, st, what):
+        if st != OK:
+            detail = lib().mcb_batch_last_error(self._h)
+            raise McbError(st, f"{what}: {status_string(st)}" + (f": {detail.decode()}" if detail else ""))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mcb_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, params, bubbles, width=3.0):
+        arr = _bubble_array(bubbles)
+        cid = ctypes.c_int()
+        self._chk(lib().mcb_batch_add(self._h, ctypes.byref(params), len(arr), arr, width, ctypes.byref(cid)),
+                  "mcb_batch_add")
+        self.shapes[cid.value] = (params.nz, params.ny, params.nx)
+        return cid.value
+
+    def add_case(self, case, **overrides):
+        p = case.params
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        return self.add(p, case_bubbles(case), case.width)
+
+    def remove(self, cid):
+        self._chk(lib().mcb_batch_remove(self._h, cid), "mcb_batch_remove")
+        self.shapes.pop(cid, None)
+
+    def step(self, n=1):
+        self._chk(lib().mcb_batch_step(self._h, int(n)), "mcb_batch_step")
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        self._chk(lib().mcb_batch_step_timed(self._h, int(n), ctypes.byref(ms)), "mcb_batch_step_timed")
+        return ms.value
+
+    def compute_macros(self, cid):
+        self._chk(lib().mcb_batch_compute_macros(self._h, cid), "mcb_batch_compute_macros")
+
+    def read(self, cid, name):
+        shape = self.shapes[cid]
+        fid = FIELDS[name]
+        cells = shape[0] * shape[1] * shape[2]
+        n = 19 * cells if fid >= 16 else cells
+        out = np.empty(n, dtype=np.float32)
+        self._chk(lib().mcb_batch_read_field(self._h, cid, fid, _fptr(out), n), f"read {name}")
+        return out.reshape(((19,) if fid >= 16 else ()) + shape)
+
+    def write(self, cid, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).ravel()
+        self._chk(lib().mcb_batch_write_field(self._h, cid, FIELDS[name], _fptr(a), a.size), f"write {name}")
+
+    def observables(self, cid):
+        self.compute_macros(cid)
+        return {k: self.read(cid, k) for k in ("phi", "rho", "ux", "uy", "uz")}
+
+    def reduce(self, cid):
+        s, k = ctypes.c_double(), ctypes.c_double()
+        st = lib().mcb_batch_reduce(self._h, cid, ctypes.byref(s), ctypes.byref(k))
+        if st not in (OK, ERR_NONFINITE):
+            self._chk(st, "mcb_batch_reduce")
+        return s.value, k.value, st == OK
+
+    def case_steps(self, cid):
+        v = ctypes.c_longlong()
+        self._chk(lib().mcb_batch_case_steps(self._h, cid, ctypes.byref(v)), "mcb_batch_case_steps")
+        return v.value
+
+    def set_tiling(self, cid, tx, ty, zc):
+        self._chk(lib().mcb_batch_set_tiling(self._h, cid, tx, ty, zc), "mcb_batch_set_tiling")
+
+    def synchronize(self):
+        self._chk(lib().mcb_batch_synchronize(self._h), "mcb_batch_synchronize")
+
+    def __len__(self):
+        return lib().mcb_batch_count(self._h)
+
+    def nbytes(self):
+        return lib().mcb_batch_bytes(self._h)
+
+
+def serve_trace(n, rate_per_s, step_scale=1.0, seed=0):
+    """Requests of the serving workload with Poisson arrivals (mcb_serve_trace)."""
+    reqs = (Request * n)()
+    check(lib().mcb_serve_trace(ctypes.c_uint64(seed), n, rate_per_s, step_scale, reqs), what="mcb_serve_trace")
+    return reqs
+
+
+def serve(reqs, *, storage=STORE_FP16, sequential=False, budget=0, max_cases=0, tick_steps=0, device=0):
+    """Replay a request trace (mcb_serve_run); returns (results array, stats)."""
+    cfg = ServeConfig()
+    cfg.device, cfg.storage, cfg.hbm_budget = device, storage, budget
+    cfg.max_cases, cfg.tick_steps, cfg.sequential = max_cases, tick_steps, int(sequential)
+    n = len(reqs)
+    res = (RequestResult * n)()
+    st = ServeStats()
+    check(lib().mcb_serve_run(ctypes.byref(cfg), reqs, n, res, ctypes.byref(st)), what="mcb_serve_run")
+    return res, st
 
 
 def tilings():

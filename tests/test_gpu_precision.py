@@ -1,0 +1,134 @@
+"""16-bit population storage (FP16 / BF16 offsets, FP32 arithmetic) against
+an FP64 run of the oracle.
+
+The model is chaotic at the rounding level (a one-ulp change grows to
+percent-level differences in u over 1000 steps), so the per-field error of a
+reduced-precision run is judged against the FP32 transcription's own distance
+from FP64 on the same case: the 16-bit forms may be at most RATIO times
+further from FP64 than FP32 is (plus a small floor). The physical observable,
+the Laplace pressure jump, must agree with FP64 to 1 %, and the phase must be
+conserved.
+"""
+import numpy as np
+import pytest
+
+from conftest import f32_bubbles, oracle_for, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CS2 = 1.0 / 3.0
+DP_TOL = 0.01          # |dp / dp_fp64 - 1|
+RATIO, FLOOR = 4.0, 2e-3
+SUM_PHI_TOL = 1e-6     # relative change of sum(phi) over the run
+
+# (n, R, sigma, tau_f, tau_g, steps): two Laplace cases (near equilibrium) and
+# the 64^3 plain-method case.
+CASES = [
+    (48, 10.0, 0.1, 1.0, 0.8, 800),
+    (48, 10.0, 0.01, 1.0, 0.8, 800),
+    (64, 10.0, 0.1, 0.55, 0.8, 1000),
+]
+
+
+def delta_p(rho, c, R, width=3.0):
+    rho = rho.astype(np.float64)
+    nz, ny, nx = rho.shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    mi = lambda d, n: (d + n / 2) % n - n / 2
+    r = np.sqrt(mi(x - c[0], nx) ** 2 + mi(y - c[1], ny) ** 2 + mi(z - c[2], nz) ** 2)
+    return CS2 * (rho[r < R - 2 * width].mean() - rho[r > R + 2 * width].mean())
+
+
+def errors(got, ref):
+    u_ref = np.sqrt(sum(ref[k] ** 2 for k in ("ux", "uy", "uz")))
+    du = np.sqrt(sum(np.sum((got[k].astype(np.float64) - ref[k]) ** 2) for k in ("ux", "uy", "uz")))
+    return {"phi": rel_l2(got["phi"], ref["phi"]),
+            "rho": rel_l2(got["rho"].astype(np.float64) - 1.0, ref["rho"] - 1.0),
+            "u": float(du / np.linalg.norm(u_ref))}
+
+
+_cache = {}
+
+
+def fp64_and_fp32(mcb, case):
+    if case not in _cache:
+        n, R, sigma, tf, tg, steps = case
+        c = (n / 2, n / 2, n / 2)
+        p = mcb.make_params(n, n, n, tau_f=tf, tau_g=tg, sigma=sigma, path=mcb.PATH_FUSED)
+        o = oracle_for(p, prec=64)
+        o.init(f32_bubbles([(*c, R)]), 3.0)
+        o.step(steps)
+        o.macros()
+        ref = {k: o.field(k).astype(np.float64) for k in ("phi", "rho", "ux", "uy", "uz")}
+        lat = mcb.Lattice(p)
+        lat.init_bubbles([(*c, R)], 3.0)
+        lat.step(steps)
+        e32 = errors(lat.observables(), ref)
+        lat.close()
+        _cache[case] = (ref, delta_p(ref["rho"], c, R), e32)
+    return _cache[case]
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c[0]}-sigma{c[2]}")
+def test_16bit_storage_against_fp64(gpu_lib, storage, case):
+    mcb = gpu_lib
+    n, R, sigma, tf, tg, steps = case
+    c = (n / 2, n / 2, n / 2)
+    ref, dp64, e32 = fp64_and_fp32(mcb, case)
+    st = mcb.STORE_FP16 if storage == "fp16" else mcb.STORE_BF16
+    p = mcb.make_params(n, n, n, tau_f=tf, tau_g=tg, sigma=sigma, storage=st)
+    lat = mcb.Lattice(p)
+    lat.init_bubbles([(*c, R)], 3.0)
+    lat.observables()
+    s0, _, ok0 = lat.reduce()
+    lat.step(steps)
+    got = lat.observables()
+    s1, _, ok1 = lat.reduce()
+    lat.close()
+    assert ok0 and ok1
+    e = errors(got, ref)
+    dp = delta_p(got["rho"], c, R)
+    print(f"{storage} {case}: errors {e}, fp32 errors {e32}, dp {dp:.6e} fp64 {dp64:.6e}, "
+          f"sum phi rel change {(s1 - s0) / s0:.2e}")
+    assert abs(dp / dp64 - 1) <= DP_TOL, f"Laplace jump {dp} vs FP64 {dp64}"
+    assert abs(s1 - s0) / s0 <= SUM_PHI_TOL
+    for k in e:
+        assert e[k] <= RATIO * e32[k] + FLOOR, f"{k}: {e[k]:.3e} vs FP32's {e32[k]:.3e}"
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_16bit_populations_decode_to_the_fp32_state(gpu_lib, storage):
+    """Right after init the decoded populations are the reference's initial
+    populations (w_l and w_l phi) to 16-bit offset precision, and writing
+    them back and reading again changes nothing beyond that precision."""
+    mcb = gpu_lib
+    st = mcb.STORE_FP16 if storage == "fp16" else mcb.STORE_BF16
+    shape = (24, 20, 16)
+    p32 = mcb.make_params(*shape, tau_f=0.6, tau_g=0.8)
+    p16 = mcb.make_params(*shape, tau_f=0.6, tau_g=0.8, storage=st)
+    bub = [(12.2, 9.7, 8.1, 5.0)]
+    a, b = mcb.Lattice(p32), mcb.Lattice(p16)
+    a.init_bubbles(bub, 3.0)
+    b.init_bubbles(bub, 3.0)
+    # Links 1..18 within the 16-bit rounding of their offsets; the rest link
+    # also carries the cell's residual of the other 18 links.
+    tol = (2e-4 if storage == "fp16" else 2e-3) / 18.0
+    tol0 = 18 * tol
+
+    def close(x, y, what):
+        assert np.max(np.abs(x[1:] - y[1:])) <= tol, what
+        assert np.max(np.abs(x[0] - y[0])) <= tol0, what
+
+    for name in ("f", "g"):
+        close(a.read(name), b.read(name), name)
+    for name in ("f", "g"):
+        y = b.read(name)
+        b.write(name, y)
+        close(b.read(name), y, f"{name} written back")
+    a.step(6)
+    b.step(6)
+    ga, gb = a.observables(), b.observables()
+    assert rel_l2(gb["phi"], ga["phi"]) < 1e-3
+    a.close()
+    b.close()
