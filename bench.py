@@ -9,6 +9,11 @@ lattice (phase recovery pass + fused collision pass). The timed region is K
 steps with all state resident in HBM, bracketed by device synchronisation;
 HIP events around each launch inside it give the per-kernel durations for
 the roofline. Prints one JSON line.
+
+Beside the plain line (unless --plain-only): the same case with FP16 and
+BF16 population storage; the uneven 64-case batch advanced together vs one
+case at a time; and the serving trace (Poisson arrivals, FP16 continuous
+batching) vs the plain FP32 path serving one request at a time.
 """
 import argparse
 import json
@@ -21,8 +26,13 @@ sys.path.insert(0, os.path.join(ROOT, "multic-bubble-cuda_amd"))
 
 HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E peak
 B_ALG_STEP = 2 * 19 * 2 * 4           # f and g, 19 pops, read + write, FP32: 304 B per lattice update
+# Stored g populations (tau_g != 1):
 B_ALG_COLLIDE = 3 * 19 * 4            # k_collide: f read, f write, g write: 228 B per cell
 B_ALG_PHASE = 19 * 4                  # k_phase: g read: 76 B per cell
+# g in moment form (tau_g = 1, the benchmark case): g*_l(y) is recomputed from
+# phi, u, n of its source cell instead of being stored.
+B_ALG_COLLIDE_GM = 2 * 19 * 4 + 6 * 4 + 4   # f read, f write, u and n write, phi read: 180 B per cell
+B_ALG_PHASE_GM = 7 * 4 + 4                  # phi, u, n read; phi write: 32 B per cell
 METRIC = "MLUPS at 256³ + % of HBM roofline; serving p50/p99 latency vs plain solves"
 
 
@@ -46,6 +56,75 @@ def cpu_baseline(case, seconds_target=15.0):
             "sample": f"{n} steps of the same 256^3 case after 1 warm-up step, OpenMP over {threads} threads"}
 
 
+def storage_form(mcb, case, storage, steps, warmup):
+    """MLUPS of the same 256^3 case with 16-bit population storage."""
+    cells = case.params.nx * case.params.ny * case.params.nz
+    lat = mcb.Lattice.from_case(case, storage=storage, path=mcb.PATH_FUSED)
+    lat.step(warmup)
+    ms = lat.step_timed(steps) / steps
+    lat.observables()
+    s, k, finite = lat.reduce()
+    tiling = lat.tiling()
+    lat.close()
+    mlups = cells / (ms * 1e-3) / 1e6
+    return {"mlups": round(mlups, 1), "ms_per_step": round(ms, 4), "tiling": list(tiling),
+            "b_alg_per_lu": B_ALG_STEP // 2,
+            "roofline_frac": round(mlups * 1e6 * (B_ALG_STEP // 2) / 1e9 / HBM_PEAK_GBS, 4),
+            "sum_phi": s, "finite": finite}
+
+
+def batching(mcb, n_cases=64, steps=100, seed=1000):
+    """The uneven batch: n_cases FP32 cases (sides 32..128) advanced together
+    (one launch per pass per tile shape, graph-replayed) vs the same cases
+    advanced one at a time on single-case contexts."""
+    cases = [mcb.case_preset("batch", seed + i) for i in range(n_cases)]
+    sides = sorted({c.params.nx for c in cases})
+    tuned = {}
+    for s in sides:
+        tuned[s] = list(mcb.autotune(cases[[c.params.nx for c in cases].index(s)].params)[:3])
+    cells = sum(c.params.nx * c.params.ny * c.params.nz for c in cases)
+    with mcb.Batch(mcb.STORE_FP32) as b:
+        for c in cases:
+            b.add_case(c)
+        b.step(4)
+        ms_batched = b.step_timed(steps)
+        nbytes = b.nbytes()
+    ms_single = 0.0
+    for c in cases:
+        lat = mcb.Lattice.from_case(c, path=mcb.PATH_FUSED)
+        tx, ty, zc = tuned[c.params.nx]
+        lat.set_tiling(tx, ty, zc)
+        lat.step(2)
+        ms_single += lat.step_timed(steps)
+        lat.close()
+    agg = cells * steps / (ms_batched * 1e-3) / 1e6
+    seq = cells * steps / (ms_single * 1e-3) / 1e6
+    return {"cases": n_cases, "cells": cells, "steps": steps, "sides": sides, "autotuned": tuned,
+            "mlups_batched": round(agg, 1), "mlups_one_at_a_time": round(seq, 1),
+            "speedup": round(agg / seq, 3), "device_bytes": nbytes,
+            "workload": "%d 'batch' presets (seeds %d..%d): cube sides in {32,48,64,96,128}, R in [0.15,0.3] side, "
+                        "sigma in [0.01,0.1], tau_f in [0.52,0.8], FP32; %d timed steps (the preset's 2000 "
+                        "shortened for the benchmark)" % (n_cases, seed, seed + n_cases - 1, steps)}
+
+
+def serving(mcb, n=500, rate=60.0, step_scale=0.1, seed=0):
+    """The serving trace replayed twice: FP16 continuous batching, and the
+    plain FP32 path one request at a time."""
+    out = {"requests": n, "arrival_rate_per_s": rate, "step_scale": step_scale,
+           "workload": "%d 'serve' presets with Poisson arrivals at %g/s: sides 48..192 step 16, R in "
+                       "[0.1,0.3] side, sigma in [0.01,0.1], tau_f in [0.52,0.8], steps log-uniform "
+                       "[500,5000] x %g" % (n, rate, step_scale)}
+    for name, storage, seq in (("batched_fp16", mcb.STORE_FP16, False), ("sequential_fp32", mcb.STORE_FP32, True)):
+        reqs = mcb.serve_trace(n, rate, step_scale, seed)
+        res, st = mcb.serve(reqs, storage=storage, sequential=seq)
+        out[name] = {"mlups": round(st.mlups, 1), "p50_s": round(st.p50_s, 4), "p99_s": round(st.p99_s, 4),
+                     "mean_s": round(st.mean_s, 4), "makespan_s": round(st.makespan_s, 3),
+                     "completed": st.completed, "rejected": st.rejected, "peak_cases": st.peak_cases,
+                     "rounds": st.rounds}
+    out["throughput_vs_sequential"] = round(out["batched_fp16"]["mlups"] / out["sequential_fp32"]["mlups"], 3)
+    return out
+
+
 def traffic_for(workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -62,6 +141,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the plain FP32 256^3 line (no 16-bit, batching or serving sections)")
+    ap.add_argument("--serve-scale", type=float, default=0.1,
+                    help="step-count scale of the serving trace (1 = the full 500..5000 steps)")
+    ap.add_argument("--serve-rate", type=float, default=60.0, help="serving arrival rate, requests per second")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (byte-counter calibration)")
     args = ap.parse_args()
@@ -93,8 +177,11 @@ def main():
     mlups = cells * args.steps / wall / 1e6
     avg_collide_ms = t_collide / args.steps
     avg_phase_ms = t_phase / args.steps
-    achieved = B_ALG_COLLIDE * cells / (avg_collide_ms * 1e-3) / 1e9
-    phase_gbs = B_ALG_PHASE * cells / (avg_phase_ms * 1e-3) / 1e9
+    moment_form = p.tau_g == 1.0
+    b_collide = B_ALG_COLLIDE_GM if moment_form else B_ALG_COLLIDE
+    b_phase = B_ALG_PHASE_GM if moment_form else B_ALG_PHASE
+    achieved = b_collide * cells / (avg_collide_ms * 1e-3) / 1e9
+    phase_gbs = b_phase * cells / (avg_phase_ms * 1e-3) / 1e9
 
     lat.compute_macros()
     sum_phi, kinetic, finite = lat.reduce()
@@ -116,7 +203,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "256^3 periodic D3Q19 f+g, two seeded bubbles R in [16,32] (seed %d), sigma 0.1, "
-                        "tau_f 0.52, tau_g 1, FP32 populations" % args.seed,
+                        "tau_f 0.52, tau_g 1, FP32 f populations, g in moment form" % args.seed,
             "lattice": [p.nx, p.ny, p.nz],
             "storage": "fp32",
             "path": "fused (k_phase + k_collide)",
@@ -130,18 +217,27 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_for(workload),
-            "alg_bytes_per_launch": B_ALG_COLLIDE * cells,
+            "alg_bytes_per_launch": b_collide * cells,
             "avg_launch_ms": round(avg_collide_ms, 5),
         },
         "step_roofline": {
             "b_alg_per_lu": B_ALG_STEP,
             "achieved_gbs": round(mlups * 1e6 * B_ALG_STEP / 1e9, 1),
             "frac": round(mlups * 1e6 * B_ALG_STEP / 1e9 / HBM_PEAK_GBS, 4),
+            "g_storage": "moments (phi, u, n)" if moment_form else "populations",
+            "b_moved_per_lu": b_collide + b_phase,
+            "moved_frac": round(mlups * 1e6 * (b_collide + b_phase) / 1e9 / HBM_PEAK_GBS, 4),
             "k_phase_ms": round(avg_phase_ms, 5),
             "k_phase_gbs": round(phase_gbs, 1),
         },
         "diagnostics": {"sum_phi": sum_phi, "kinetic": kinetic, "finite": finite},
     }
+    if not args.plain_only:
+        for name, st in (("fp16_storage", mcb.STORE_FP16), ("bf16_storage", mcb.STORE_BF16)):
+            out[name] = storage_form(mcb, case, st, args.steps, args.warmup)
+            out[name]["vs_plain"] = round(out[name]["mlups"] / out["value"], 3)
+        out["batching"] = batching(mcb)
+        out["serving"] = serving(mcb, rate=args.serve_rate, step_scale=args.serve_scale)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(out), flush=True)

@@ -61,6 +61,12 @@ struct mcb_ctx {
     std::unique_ptr<Batch> batch;
     int case_id = 0;
     bool initialised = false;
+    // FUSED FP32 at tau_g = 1: g in moment form (GMode), phi double-buffered
+    // (phis[cur] is written by the next phase pass, phis[cur ^ 1] is the state).
+    int gm = GM_POPULATIONS;
+    float* phis[2] = {nullptr, nullptr};
+    float* gmu[3] = {nullptr, nullptr, nullptr};
+    float* gmn[3] = {nullptr, nullptr, nullptr};
 };
 
 struct mcb_batch {
@@ -115,7 +121,8 @@ size_t footprint(const mcb_params& p, int path) {
     const long long n = (long long)p.nx * p.ny * p.nz;
     const long long P = plane_stride(n);
     size_t floats = 0;
-    if (path == MCB_PATH_FUSED) floats = 4 * 19 * (size_t)P + 11 * (size_t)P;
+    if (path == MCB_PATH_FUSED && p.tau_g == 1.0f) floats = 2 * 19 * (size_t)P + 18 * (size_t)P;   // g in moment form
+    else if (path == MCB_PATH_FUSED) floats = 4 * 19 * (size_t)P + 11 * (size_t)P;
     else floats = 3 * 19 * (size_t)P + 19 * (size_t)P;
     return floats * sizeof(float) + (2 * kReduceBlocks + 2) * sizeof(double);
 }
@@ -139,6 +146,16 @@ CaseView fused_view(mcb_ctx* c) {
     v.gsrc = c->g[c->cur];
     v.gdst = c->g[c->cur ^ 1];
     v.phi = c->phi;
+    v.gm = c->gm;
+    if (c->gm != GM_POPULATIONS) {
+        v.gsrc = v.gdst = nullptr;
+        v.phi = c->phis[c->cur];
+        v.phi_prev = c->phis[c->cur ^ 1];
+        for (int k = 0; k < 3; ++k) {
+            v.gm_u[k] = c->gmu[k];
+            v.gm_n[k] = c->gmn[k];
+        }
+    }
     v.rho = c->rho; v.ux = c->ux; v.uy = c->uy; v.uz = c->uz;
     v.fx = c->ffx; v.fy = c->ffy; v.fz = c->ffz;
     v.nx = c->normx; v.ny = c->normy; v.nz = c->normz;
@@ -147,11 +164,44 @@ CaseView fused_view(mcb_ctx* c) {
 
 int step_mode(const mcb_ctx* c) { return c->p.tau_g == 1.0f ? COLLIDE_STEP : COLLIDE_STEP_BGK; }
 
+// Fused path, before the initial phi is written: A/B parity 0, and in moment
+// form phi0 goes to the state buffer phis[1].
+void fused_init_target(mcb_ctx* c) {
+    c->cur = 0;
+    if (c->g[0] == nullptr) {
+        c->gm = GM_INITIAL;
+        c->phi = c->phis[1];
+    }
+}
+
+// Fused path, after the initial phi is in c->phi: f*_l = w_l, and g* =
+// w_l phi0(y + c_l) in pull form or nothing (initDist is implied by
+// GM_INITIAL) in moment form; the macro fields start at zero.
+hipError_t fused_init_state(mcb_ctx* c) {
+    const size_t pbytes = (size_t)c->geo.plane * sizeof(float);
+    float* zero[] = {c->rho, c->ux, c->uy, c->uz, c->ffx, c->ffy, c->ffz, c->normx, c->normy, c->normz};
+    for (float* q : zero) {
+        hipError_t e = hipMemsetAsync(q, 0, pbytes, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return launch_init_pull(c->phi, c->f[0], c->g[0], c->geo, c->phys, c->stream);
+}
+
+// After a fused step: flip the A/B buffers; in moment form the phi just
+// written becomes the state's phi.
+void after_fused_step(mcb_ctx* c) {
+    c->cur ^= 1;
+    if (c->gm != GM_POPULATIONS) {
+        c->gm = GM_MOMENTS;
+        c->phi = c->phis[c->cur ^ 1];
+    }
+}
+
 mcb_status fused_step(mcb_ctx* c) {
     const CaseView v = fused_view(c);
     MCB_HIP(c, launch_phase(v, c->stream));
     MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
-    c->cur ^= 1;
+    after_fused_step(c);
     return MCB_OK;
 }
 
@@ -309,14 +359,24 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
         c->tiling = choose_tiling(*p);
     } else if (path == MCB_PATH_FUSED) {
         c->geo.plane = P;
+        // At tau_g = 1 g is kept in moment form: phi (two buffers), u and n
+        // instead of two sets of 19 planes.
+        const bool moments = p->tau_g == 1.0f;
         for (int b = 0; b < 2; ++b) {
             if ((st = alloc(c, &c->f[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
-            if ((st = alloc(c, &c->g[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
+            if (!moments && (st = alloc(c, &c->g[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
         }
         float** scal[] = {&c->phi, &c->rho, &c->ux, &c->uy, &c->uz, &c->ffx, &c->ffy, &c->ffz,
                           &c->normx, &c->normy, &c->normz};
         for (auto* q : scal)
             if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
+        if (moments) {
+            c->gm = GM_INITIAL;
+            c->phis[0] = c->phi;
+            float** gmf[] = {&c->phis[1], &c->gmu[0], &c->gmu[1], &c->gmu[2], &c->gmn[0], &c->gmn[1], &c->gmn[2]};
+            for (auto* q : gmf)
+                if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
+        }
         c->tiling = choose_tiling(*p);
     } else {
         // The reference layout: planes of exactly n cells (src/header.cuh:21).
@@ -383,12 +443,9 @@ mcb_status ctx_init_bubbles_device(mcb_ctx* c, const BubbleSet& s) {
     }
     if (c->path != MCB_PATH_FUSED || c->p.bc_mode != MCB_BC_PERIODIC)
         return fail(c, MCB_ERR_UNSUPPORTED, "device initialisation needs the periodic fused path");
+    fused_init_target(c);
     MCB_HIP(c, launch_bubble_phi(c->phi, c->geo, s, c->stream));
-    c->cur = 0;
-    const size_t pbytes = (size_t)c->geo.plane * sizeof(float);
-    float* zero[] = {c->rho, c->ux, c->uy, c->uz, c->ffx, c->ffy, c->ffz, c->normx, c->normy, c->normz};
-    for (float* q : zero) MCB_HIP(c, hipMemsetAsync(q, 0, pbytes, c->stream));
-    MCB_HIP(c, launch_init_pull(c->phi, c->f[0], c->g[0], c->geo, c->phys, c->stream));
+    MCB_HIP(c, fused_init_state(c));
     MCB_HIP(c, hipStreamSynchronize(c->stream));
     c->initialised = true;
     return MCB_OK;
@@ -414,14 +471,12 @@ mcb_status mcb_init_phi(mcb_ctx* c, const float* phi_host, size_t n) {
     }
     const size_t bytes = (size_t)c->geo.n * sizeof(float);
     const int NX = c->p.nx, NY = c->p.ny, NZ = c->p.nz;
-    MCB_HIP(c, hipMemcpyAsync(c->phi, phi_host, bytes, hipMemcpyHostToDevice, c->stream));
     if (c->path == MCB_PATH_FUSED) {
-        c->cur = 0;
-        const size_t pbytes = (size_t)c->geo.plane * sizeof(float);
-        float* zero[] = {c->rho, c->ux, c->uy, c->uz, c->ffx, c->ffy, c->ffz, c->normx, c->normy, c->normz};
-        for (float* q : zero) MCB_HIP(c, hipMemsetAsync(q, 0, pbytes, c->stream));
-        MCB_HIP(c, launch_init_pull(c->phi, c->f[0], c->g[0], c->geo, c->phys, c->stream));
+        fused_init_target(c);
+        MCB_HIP(c, hipMemcpyAsync(c->phi, phi_host, bytes, hipMemcpyHostToDevice, c->stream));
+        MCB_HIP(c, fused_init_state(c));
     } else {
+        MCB_HIP(c, hipMemcpyAsync(c->phi, phi_host, bytes, hipMemcpyHostToDevice, c->stream));
         // src/var.cu:86-101 clears these; initTensor/initDist set the rest.
         float* zero[] = {c->ux, c->uy, c->uz, c->normx, c->normy, c->normz, c->curvature, c->indicator,
                          c->ffx, c->ffy, c->ffz};
@@ -464,6 +519,7 @@ mcb_status mcb_compute_macros(mcb_ctx* c) {
         const CaseView v = fused_view(c);
         MCB_HIP(c, launch_phase(v, c->stream));
         MCB_HIP(c, launch_collide(v, COLLIDE_MACROS, c->tiling, c->stream));
+        if (c->gm != GM_POPULATIONS) c->phi = v.phi;   // the phi just computed (not the state's)
         return MCB_OK;
     }
     return staged_macros(c);
@@ -482,9 +538,14 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
             MCB_HIP(c, hipMemcpyAsync(host, src, 19 * cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         } else {
             // Gather into the idle A/B buffer, then copy plane by plane.
-            const float* src = field == MCB_FIELD_F ? c->f[c->cur] : c->g[c->cur];
-            float* tmp = field == MCB_FIELD_F ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1];
-            MCB_HIP(c, launch_link_shift(src, tmp, c->geo, +1, c->stream));
+            if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
+                MCB_HIP(c, launch_gm_to_ref(fused_view(c), c->f[c->cur ^ 1], c->stream));
+            } else {
+                const float* src = field == MCB_FIELD_F ? c->f[c->cur] : c->g[c->cur];
+                MCB_HIP(c, launch_link_shift(src, field == MCB_FIELD_F ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1], c->geo,
+                                             +1, c->stream));
+            }
+            const float* tmp = field == MCB_FIELD_F || c->gm != GM_POPULATIONS ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1];
             for (int l = 0; l < 19; ++l)
                 MCB_HIP(c, hipMemcpyAsync(host + l * cells, tmp + (size_t)l * c->geo.plane, cells * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream));
@@ -511,6 +572,16 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             float* dst = field == MCB_FIELD_F ? c->sf : c->sg;
             MCB_HIP(c, hipMemcpyAsync(dst, host, 19 * cells * sizeof(float), hipMemcpyHostToDevice, c->stream));
         } else {
+            if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
+                // Arbitrary g populations have no moment form: from here on
+                // the context keeps g as populations (phi as its one buffer).
+                for (int b = 0; b < 2; ++b) {
+                    mcb_status as = alloc(c, &c->g[b], 19 * (size_t)c->geo.plane);
+                    if (as != MCB_OK) return as;
+                }
+                c->phi = c->phis[c->cur ^ 1];
+                c->gm = GM_POPULATIONS;
+            }
             float* dst = field == MCB_FIELD_F ? c->f[c->cur] : c->g[c->cur];
             float* tmp = field == MCB_FIELD_F ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1];
             for (int l = 0; l < 19; ++l)
@@ -519,6 +590,11 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             MCB_HIP(c, launch_link_shift(tmp, dst, c->geo, -1, c->stream));
         }
     } else {
+        if (field == MCB_FIELD_PHI && c->gm != GM_POPULATIONS && n == cells) {
+            // phi is part of the moment-form state: write the buffer the next
+            // phase pass recomputes, as writing phi does with populations.
+            c->phi = c->phis[c->cur];
+        }
         float* dst = scalar_field(c, field);
         if (!dst) return fail(c, MCB_ERR_INVALID_ARG, "unknown field");
         if (n != cells) return fail(c, MCB_ERR_INVALID_ARG, "field buffer must hold nx*ny*nz floats");
@@ -575,7 +651,7 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
         MCB_HIP(c, hipEventRecord(e[1], c->stream));
         MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[2], c->stream));
-        c->cur ^= 1;
+        after_fused_step(c);
     }
     MCB_HIP(c, hipEventSynchronize(c->tev[need - 1]));
     double acc[2] = {0.0, 0.0};

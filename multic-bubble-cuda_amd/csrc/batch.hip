@@ -62,6 +62,13 @@ CaseView CaseState::view(int q) const {
     v.mslot = q;
     v.mass = mass;
     v.partial = partial;
+    // Moment form: only the parity of the initial state (cur = 0 before the
+    // first step) sees GM_INITIAL.
+    v.gm = gm == GM_INITIAL && q != cur ? GM_MOMENTS : gm;
+    for (int k = 0; k < 3; ++k) {
+        v.gm_u[k] = gmu[k];
+        v.gm_n[k] = gmn[k];
+    }
     return v;
 }
 
@@ -236,13 +243,25 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     };
     mcb_status st;
     const size_t pops = 19 * (size_t)P * pop_bytes(p.storage);
+    // FP32 at tau_g = 1 keeps g in moment form (phi, u, n of the last
+    // collision) instead of two sets of 19 planes.
+    const bool moments = p.storage == MCB_STORE_FP32 && c->mode == COLLIDE_STEP;
     for (int b = 0; b < 2; ++b) {
         if ((st = alloc(pops, &c->f[b])) != MCB_OK) return st;
-        if ((st = alloc(pops, &c->g[b])) != MCB_OK) return st;
+        if (!moments && (st = alloc(pops, &c->g[b])) != MCB_OK) return st;
     }
     void* q = nullptr;
     if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
     c->phi[0] = c->phi[1] = static_cast<float*>(q);
+    if (moments) {
+        if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
+        c->phi[1] = static_cast<float*>(q);
+        for (int k = 0; k < 6; ++k) {
+            if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
+            (k < 3 ? c->gmu[k] : c->gmn[k - 3]) = static_cast<float*>(q);
+        }
+        c->gm = GM_INITIAL;
+    }
     if (p.storage != MCB_STORE_FP32) {
         if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
         c->phi[1] = static_cast<float*>(q);
@@ -269,7 +288,8 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
         }
     }
     if (p.storage == MCB_STORE_FP32) {
-        B_HIP(launch_init_pull(c->phi[0], static_cast<float*>(c->f[0]), static_cast<float*>(c->g[0]), c->geo, c->phys,
+        // (moment form: g[0] is null and phi[1] keeps phi0 as the state)
+        B_HIP(launch_init_pull(c->phi[1], static_cast<float*>(c->f[0]), static_cast<float*>(c->g[0]), c->geo, c->phys,
                                stream_));
     } else {
         B_HIP(launch_init16(c->phi[1], c->f[0], c->g[0], c->rho_anchor[1], c->geo, c->phys, p.storage, stream_));
@@ -370,30 +390,31 @@ mcb_status Batch::rebuild() {
         for (auto& v : d_views_) B_HIP(hipMalloc(reinterpret_cast<void**>(&v), cap * sizeof(CaseView)));
         views_cap_ = cap;
     }
-    mode_ = COLLIDE_STEP;
-    std::vector<CaseView> hv[2];
-    std::map<std::pair<int, int>, std::vector<WorkItem>> by_shape;
+    std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_group;
+    any_initial_ = false;
+    for (int q = 0; q < 2; ++q) hv_[q].clear();
     for (size_t i = 0; i < n; ++i) {
         CaseState* c = cases_[order_[i]].get();
-        if (c->mode == COLLIDE_STEP_BGK) mode_ = COLLIDE_STEP_BGK;
         for (int q = 0; q < 2; ++q) {
             CaseView v = c->view(q ^ c->parity_offset);
             float** m = reinterpret_cast<float**>(&v.rho);
             for (int k = 0; k < kMacroFields; ++k) m[k] = static_cast<float*>(macro_bufs_[k]);
-            hv[q].push_back(v);
+            hv_[q].push_back(v);
         }
+        if (c->gm == GM_INITIAL) any_initial_ = true;
         auto tl = tiles_of(*c, (int)i);
-        auto& vec = by_shape[{c->tiling.tx, c->tiling.ty}];
+        auto& vec = by_group[std::make_tuple(c->tiling.tx, c->tiling.ty, c->mode)];
         vec.insert(vec.end(), tl.begin(), tl.end());
         c->slot = (int)i;
     }
     for (int q = 0; q < 2; ++q)
-        if (n) B_HIP(hipMemcpyAsync(d_views_[q], hv[q].data(), n * sizeof(CaseView), hipMemcpyHostToDevice, stream_));
+        if (n) B_HIP(hipMemcpyAsync(d_views_[q], hv_[q].data(), n * sizeof(CaseView), hipMemcpyHostToDevice, stream_));
     for (auto& kv : groups_) kv.second.n = 0;
     std::vector<std::vector<WorkItem>> staged;
-    for (auto& kv : by_shape) {
+    for (auto& kv : by_group) {
         Group& gr = groups_[kv.first];
-        gr.t = Tiling{kv.first.first, kv.first.second, 0};
+        gr.t = Tiling{std::get<0>(kv.first), std::get<1>(kv.first), 0};
+        gr.mode = std::get<2>(kv.first);
         staged.push_back(xcd_permute(kv.second));
         const auto& items = staged.back();
         if (items.size() > gr.cap) {
@@ -449,7 +470,7 @@ mcb_status Batch::launch_step(int q) {
     }
     for (auto& kv : groups_) {
         const Group& gr = kv.second;
-        if (gr.n) B_HIP(launch_collide_batch(d_views_[q], gr.d_work, gr.n, gr.t, mode_, storage_, stream_));
+        if (gr.n) B_HIP(launch_collide_batch(d_views_[q], gr.d_work, gr.n, gr.t, gr.mode, storage_, stream_));
     }
     return MCB_OK;
 }
@@ -464,6 +485,24 @@ mcb_status Batch::step(int nsteps) {
     }
     if (nsteps == 0) return MCB_OK;
     int left = nsteps;
+    if (any_initial_) {
+        // The first step of moment-form cases reads their initial state; the
+        // view of that parity then switches to the moment form (stream-ordered
+        // copy from the persistent host array, so graphs see it).
+        mcb_status st = launch_step(parity_);
+        if (st != MCB_OK) return st;
+        const int q = parity_;
+        parity_ ^= 1;
+        left -= 1;
+        for (size_t i = 0; i < order_.size(); ++i) {
+            CaseState* c = cases_[order_[i]].get();
+            if (c->gm != GM_INITIAL) continue;
+            c->gm = GM_MOMENTS;
+            hv_[q][i].gm = GM_MOMENTS;
+            B_HIP(hipMemcpyAsync(d_views_[q] + i, &hv_[q][i], sizeof(CaseView), hipMemcpyHostToDevice, stream_));
+        }
+        any_initial_ = false;
+    }
     while (left > 0) {
         if (use_graphs_ && graph_[parity_] && left >= 2) {
             B_HIP(hipGraphLaunch(graph_[parity_], stream_));
@@ -515,7 +554,9 @@ mcb_status Batch::read(int id, int field, float* host, size_t n) {
         float* tmp = nullptr;
         B_HIP(hipMalloc(reinterpret_cast<void**>(&tmp), 19 * P * sizeof(float)));
         hipError_t e;
-        if (c->storage == MCB_STORE_FP32)
+        if (!isf && c->gm != GM_POPULATIONS)
+            e = launch_gm_to_ref(c->view(c->cur), tmp, stream_);
+        else if (c->storage == MCB_STORE_FP32)
             e = launch_link_shift(static_cast<const float*>(isf ? c->f[c->cur] : c->g[c->cur]), tmp, c->geo, +1, stream_);
         else
             e = launch_decode16(isf ? c->f[c->cur] : c->g[c->cur], isf ? c->rho_anchor[c->cur ^ 1] : c->phi[c->cur ^ 1],
@@ -551,6 +592,20 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
     if (field == MCB_FIELD_F || field == MCB_FIELD_G) {
         if (n != 19 * cells) return fail(MCB_ERR_INVALID_ARG, "population buffer must hold 19*nx*ny*nz floats");
         const bool isf = field == MCB_FIELD_F;
+        if (!isf && c->gm != GM_POPULATIONS) {
+            // Arbitrary g populations have no moment form: the case keeps g as
+            // populations from here on.
+            B_HIP(hipStreamSynchronize(stream_));
+            const size_t pops = 19 * P * sizeof(float);
+            for (int b = 0; b < 2; ++b) {
+                mcb_status st = dev_alloc(pops, &c->g[b]);
+                if (st != MCB_OK) return st;
+                c->allocs.emplace_back(c->g[b], pops);
+                c->bytes += pops;
+            }
+            c->gm = GM_POPULATIONS;
+            dirty_ = true;
+        }
         float* tmp = nullptr;
         B_HIP(hipMalloc(reinterpret_cast<void**>(&tmp), 19 * P * sizeof(float)));
         hipError_t e = hipSuccess;
@@ -578,6 +633,9 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
     if (c->storage != MCB_STORE_FP32)
         return fail(MCB_ERR_UNSUPPORTED, "with 16-bit populations phi anchors the g populations; write G instead");
     if (n != cells) return fail(MCB_ERR_INVALID_ARG, "field buffer must hold nx*ny*nz floats");
+    // In moment form phi of the state is phi[cur ^ 1]; like a write with
+    // populations, the written phi is what the next phase pass replaces.
+    if (c->gm != GM_POPULATIONS) c->phi_last = c->phi[c->cur];
     B_HIP(hipMemcpyAsync(c->phi_last, host, cells * sizeof(float), hipMemcpyHostToDevice, stream_));
     B_HIP(hipStreamSynchronize(stream_));
     return MCB_OK;

@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -34,6 +35,9 @@ struct CaseState {
     float* phi[2] = {nullptr, nullptr};
     float* rho_anchor[2] = {nullptr, nullptr};
     float* phi_last = nullptr;      // written by the most recent phase pass
+    int gm = GM_POPULATIONS;        // FP32 at tau_g = 1: g in moment form (GMode)
+    float* gmu[3] = {nullptr, nullptr, nullptr};
+    float* gmn[3] = {nullptr, nullptr, nullptr};
     MassCell* mass = nullptr;       // 16-bit storage: mass-drift correction state
     double* partial = nullptr;      // one FP64 partial per phase work item
     size_t partial_cap = 0;
@@ -119,18 +123,21 @@ private:
     int parity_ = 0;
     long long batch_steps_ = 0;
     bool dirty_ = true;
-    int mode_ = COLLIDE_STEP;
     int storage_ = -1;                // all cases of a batch share one storage form
     // device arrays
     CaseView* d_views_[2] = {nullptr, nullptr};
     size_t views_cap_ = 0;
     struct Group {
         Tiling t{};
+        int mode = COLLIDE_STEP;
         WorkItem* d_work = nullptr;
         size_t cap = 0;
         unsigned n = 0;
     };
-    std::map<std::pair<int, int>, Group> groups_;   // by tile shape, kept across rebuilds
+    // Launch groups by (tile x, tile y, collide mode), kept across rebuilds.
+    std::map<std::tuple<int, int, int>, Group> groups_;
+    std::vector<CaseView> hv_[2];     // host copies of the view arrays
+    bool any_initial_ = false;        // a moment-form case has not stepped yet
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     double* d_partial_ = nullptr;     // reduction scratch (2 * kReduceBlocks + 2 doubles)
     std::vector<void*> macro_bufs_;

@@ -75,7 +75,31 @@ __global__ __launch_bounds__(PBX * PBY) void k_init_pull(const float* __restrict
         constexpr int l = decltype(lc)::value;
         const float w = p.w[WC[l]];
         f[(size_t)l * G.plane + me] = w * 1.0f;
-        g[(size_t)l * G.plane + me] = w * phi[zs[1 + CZ[l]] + ys[1 + CY[l]] + xs[1 + CX[l]]];
+        if (g) g[(size_t)l * G.plane + me] = w * phi[zs[1 + CZ[l]] + ys[1 + CY[l]] + xs[1 + CX[l]]];
+    });
+}
+
+// Moment form -> the reference's post-stream g layout:
+// dst_l(x) = g*_l(x - c_l) recomputed from phi, u, n (or w_l phi0(x) before
+// the first step).
+__global__ __launch_bounds__(PBX * PBY) void k_gm_to_ref(CaseView c, float* __restrict__ dst) {
+    const Geo& G = c.geo;
+    const int x = blockIdx.x * PBX + threadIdx.x;
+    const int y = blockIdx.y * PBY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= G.nx || y >= G.ny) return;
+    const unsigned xs[3] = {(unsigned)shift<-1>(x, G.nx), (unsigned)x, (unsigned)shift<1>(x, G.nx)};
+    const unsigned ys[3] = {(unsigned)(shift<-1>(y, G.ny) * G.nx), (unsigned)(y * G.nx),
+                            (unsigned)(shift<1>(y, G.ny) * G.nx)};
+    const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                            (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+    const unsigned me = zs[1] + ys[1] + xs[1];
+    for_links([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        float v;
+        if (c.gm == GM_INITIAL) v = c.phys.w[WC[l]] * ld32(c.phi_prev, me);
+        else v = g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+        dst[(size_t)l * G.plane + me] = v;
     });
 }
 
@@ -298,6 +322,12 @@ hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, uns
 
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s) {
     hipLaunchKernelGGL(k_init_pull, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s) {
+    if (c.gm == GM_POPULATIONS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gm_to_ref, cell_grid(c.geo), dim3(PBX, PBY, 1), 0, s, c, dst);
     return hipGetLastError();
 }
 

@@ -20,6 +20,13 @@ namespace mcb {
 
 enum CollideMode { COLLIDE_STEP = 0, COLLIDE_STEP_BGK = 1, COLLIDE_MACROS = 2 };
 
+// How a case's state holds g (CaseView::gm).
+//   GM_POPULATIONS : 19 planes of g* (pull form), any tau_g.
+//   GM_MOMENTS     : phi, u, n of the last collision (tau_g = 1 only).
+//   GM_INITIAL     : the initial state g_l(x) = w_l phi0(x) (initDist), before
+//                    the first step of the moment form; phi_prev holds phi0.
+enum GMode { GM_POPULATIONS = 0, GM_MOMENTS = 1, GM_INITIAL = 2 };
+
 struct Tiling {
     int tx, ty, zc;   // x/y tile of a workgroup, z cells marched per workgroup
 };
@@ -50,9 +57,17 @@ struct CaseView {
     const void* gsrc;
     void* gdst;
     float* phi;             // phi of this step: written by the phase pass
-    const float* phi_prev;  // 16-bit storage: phi of the previous step (anchor of g)
+    const float* phi_prev;  // 16-bit storage: phi of the previous step (anchor of g);
+                            // moment form: phi of the previous step's collision
     const float* rho_prev;  // 16-bit storage: rho of the previous step (anchor of f)
     float* rho_next;        // 16-bit storage: rho of this step
+    // Moment form of g (FP32, tau_g = 1): g*_l(y) is collisionPhase of
+    // (phi, u, n) at y, so the state keeps u and n (gm_u, gm_n) instead of
+    // 19 g planes and the phase pass recomputes g*_l(x - c_l).
+    float* gm_u[3];
+    float* gm_n[3];
+    int gm;                 // GM_POPULATIONS, GM_MOMENTS or GM_INITIAL
+    int reserved2;
     // COLLIDE_MACROS outputs
     float* rho;
     float* ux;
@@ -79,8 +94,10 @@ Tiling tiling_at(int i);
 hipError_t launch_phase(const CaseView& c, hipStream_t s);
 hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStream_t s);
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
-// initTensor + initDist (src/init.cu:21-24, 69-79).
+// initTensor + initDist (src/init.cu:21-24, 69-79). g may be null (moment form).
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s);
+// Moment form of g -> reference layout (post-stream g_l(x), 19 planes).
+hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->
 // reference layout; sign = -1: reference layout -> pull form).
 hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int sign, hipStream_t s);

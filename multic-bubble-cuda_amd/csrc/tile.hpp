@@ -116,10 +116,35 @@ template <class S> struct PopPlanes {
 
 // phiCalc (src/lbm.cu:18-25) for one cell of a pull-form state: the
 // post-stream g_l(x) is g*_l(x - c_l). xs/ys/zs: offsets of x-1, x, x+1 etc.
+//
+// Moment form (FP32, tau_g = 1): g*_l(y) is recomputed from phi, u, n of the
+// source cell with collisionPhase's own expressions (lbm.cu:319-326), so the
+// sum carries the same bits as with stored populations. The initial state is
+// initDist's g_l(x) = w_l phi0(x) (init.cu:69-79).
+template <int L> __device__ __forceinline__ float g_from_moments(const CaseView& c, unsigned y) {
+    const PhaseCell pc = phase_cell(c.phys, ld32(c.phi_prev, y), ld32(c.gm_u[0], y), ld32(c.gm_u[1], y),
+                                    ld32(c.gm_u[2], y), ld32(c.gm_n[0], y), ld32(c.gm_n[1], y), ld32(c.gm_n[2], y));
+    return phase_post<L>(pc, c.phys);
+}
+
 template <class S>
 __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const unsigned (&xs)[3],
                                            const unsigned (&ys)[3], const unsigned (&zs)[3]) {
     float sum = 0.0f;
+    if constexpr (!S::kAnchored) {
+        if (c.gm == GM_MOMENTS) {
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                sum += g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+            });
+            return sum;
+        }
+        if (c.gm == GM_INITIAL) {
+            const float p0 = ld32(c.phi_prev, zs[1] + ys[1] + xs[1]);
+            for_links([&](auto lc) { sum += c.phys.w[WC[decltype(lc)::value]] * p0; });
+            return sum;
+        }
+    }
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
@@ -354,6 +379,17 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                 constexpr int l = decltype(lc)::value;
                 fdst.template st<l>(me, fluid_post<l>(fc, c.phys, d));
             });
+            if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
+                // Moment form: keep what collisionPhase needs (phic is the
+                // phi buffer of this step, the next step's phi_prev).
+                st32(c.gm_u[0], me, m.ux);
+                st32(c.gm_u[1], me, m.uy);
+                st32(c.gm_u[2], me, m.uz);
+                st32(c.gm_n[0], me, ncx);
+                st32(c.gm_n[1], me, ncy);
+                st32(c.gm_n[2], me, ncz);
+                continue;
+            }
             for_links([&](auto lc) {
                 constexpr int l = decltype(lc)::value;
                 if constexpr (MODE == COLLIDE_STEP) {

@@ -103,9 +103,13 @@ def test_unsupported_combinations():
 
 
 def test_footprint():
-    p = mcb.make_params(64, 64, 64)
+    n = 64 ** 3
+    p = mcb.make_params(64, 64, 64, tau_g=0.8)
     fused = mcb.lib().mcb_footprint_bytes(ctypes.byref(p))
-    assert fused >= 4 * 19 * 64 ** 3 * 4
+    assert fused >= 4 * 19 * n * 4                      # f and g, two buffers each
+    p.tau_g = 1.0
+    moments = mcb.lib().mcb_footprint_bytes(ctypes.byref(p))
+    assert 2 * 19 * n * 4 + 18 * n * 4 <= moments < fused   # g as phi, u, n
     p.path = mcb.PATH_STAGED
     assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) >= 3 * 19 * 64 ** 3 * 4
 

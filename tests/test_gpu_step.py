@@ -71,10 +71,50 @@ def test_cfg0_1000_steps(gpu_lib, path):
     lat.close()
 
 
-def test_fused_tilings_agree(gpu_lib):
+def test_moment_form_1000_steps(gpu_lib):
+    """tau_g = 1 keeps g as (phi, u, n) and recomputes g* in the phase pass:
+    1000 steps stay bit-identical to the transcription."""
+    mcb = gpu_lib
+    p = mcb.make_params(48, 48, 48, tau_f=0.55, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED)
+    bubbles = [(24.3, 23.6, 24.1, 10.0)]
+    lat, o, got, want = run_pair(mcb, p, bubbles, 1000)
+    assert_parity(got, want, "moment form 48^3 x 1000")
+    lat.close()
+
+
+def test_moment_form_populations(gpu_lib):
+    """g read from the moment form is the reference layout, before and after
+    steps; writing g switches the context to stored populations, bit for bit."""
+    mcb = gpu_lib
+    shape = (24, 20, 16)
+    p = mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, sigma=0.08, path=mcb.PATH_FUSED)
+    bubbles = [(12.0, 10.5, 8.0, 5.0)]
+    o = oracle_for(p)
+    o.init(f32_bubbles(bubbles))
+    lat = mcb.Lattice(p)
+    lat.init_bubbles(bubbles)
+    assert np.array_equal(lat.read("g").reshape(19, -1), o.field("g").reshape(19, -1)), "initial g"
+    assert np.array_equal(lat.read("phi"), o.field("phi"))
+    lat.step(5)
+    o.step(5)
+    g = lat.read("g")
+    assert np.array_equal(g.reshape(19, -1), o.field("g").reshape(19, -1)), "g after 5 steps"
+    assert np.array_equal(lat.read("f").reshape(19, -1), o.field("f").reshape(19, -1))
+    lat.write("g", g)             # now stored as populations
+    lat.step(3)
+    o.step(3)
+    got = lat.observables()
+    o.macros()
+    for k in OBS:
+        assert np.array_equal(got[k], o.field(k)), k
+    lat.close()
+
+
+@pytest.mark.parametrize("tau_g", [0.8, 1.0])
+def test_fused_tilings_agree(gpu_lib, tau_g):
     """Every tile shape and z-chunk gives the same bits."""
     mcb = gpu_lib
-    p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=0.8, sigma=0.05, path=mcb.PATH_FUSED)
+    p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=tau_g, sigma=0.05, path=mcb.PATH_FUSED)
     bubbles = [(20.5, 17.0, 14.2, 7.5)]
     ref = None
     for tx, ty in mcb.tilings():
