@@ -199,7 +199,7 @@ void after_fused_step(mcb_ctx* c) {
 
 mcb_status fused_step(mcb_ctx* c) {
     const CaseView v = fused_view(c);
-    MCB_HIP(c, launch_phase(v, c->stream));
+    MCB_HIP(c, launch_phase(v, c->tiling, c->stream));
     MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
     after_fused_step(c);
     return MCB_OK;
@@ -517,7 +517,7 @@ mcb_status mcb_compute_macros(mcb_ctx* c) {
     if (c->batch) return batch_status(c, c->batch->macros(c->case_id));
     if (c->path == MCB_PATH_FUSED) {
         const CaseView v = fused_view(c);
-        MCB_HIP(c, launch_phase(v, c->stream));
+        MCB_HIP(c, launch_phase(v, c->tiling, c->stream));
         MCB_HIP(c, launch_collide(v, COLLIDE_MACROS, c->tiling, c->stream));
         if (c->gm != GM_POPULATIONS) c->phi = v.phi;   // the phi just computed (not the state's)
         return MCB_OK;
@@ -647,7 +647,7 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
         hipEvent_t* e = &c->tev[3 * (size_t)t];
         const CaseView v = fused_view(c);
         MCB_HIP(c, hipEventRecord(e[0], c->stream));
-        MCB_HIP(c, launch_phase(v, c->stream));
+        MCB_HIP(c, launch_phase(v, c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[1], c->stream));
         MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[2], c->stream));

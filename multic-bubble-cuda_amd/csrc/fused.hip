@@ -44,6 +44,19 @@ __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, un
     st32(c.phi, zs[1] + ys[1] + xs[1], phase_sum<StoreF32>(c, gp, xs, ys, zs));
 }
 
+// Moment-form phase pass (phase_tile_gm): the collide pass's tiles and grid.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_phase_gm(CaseView c) {
+    const Geo& G = c.geo;
+    const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
+    unsigned t = xcd_order(blockIdx.x, gridDim.x);
+    const int x0 = (int)(t % gx) * TX;
+    t /= gx;
+    const int y0 = (int)(t % gy) * TY;
+    const int zb = (int)(t / gy) * c.zc;
+    phase_tile_gm<TX, TY>(c, x0, y0, zb, min(zb + c.zc, G.nz));
+}
+
 // --------------------------------------------------------------- k_collide
 
 template <int TX, int TY, int MODE>
@@ -290,7 +303,27 @@ bool tiling_supported(const Tiling& t) {
 int tiling_count() { return (int)(sizeof(kTilings) / sizeof(kTilings[0])); }
 Tiling tiling_at(int i) { return kTilings[i]; }
 
-hipError_t launch_phase(const CaseView& c, hipStream_t s) {
+hipError_t launch_phase(const CaseView& c0, const Tiling& t, hipStream_t s) {
+    if (c0.gm == GM_MOMENTS) {
+        CaseView c = c0;
+        c.zc = t.zc;
+        const unsigned tiles = (unsigned)((c.geo.nx + t.tx - 1) / t.tx) * (unsigned)((c.geo.ny + t.ty - 1) / t.ty) *
+                               (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
+#define MCB_PHASE_GM(TX, TY)                                                                        \
+    if (t.tx == TX && t.ty == TY) {                                                                 \
+        hipLaunchKernelGGL((k_phase_gm<TX, TY>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);            \
+        return hipGetLastError();                                                                   \
+    }
+        MCB_PHASE_GM(64, 8)
+        MCB_PHASE_GM(64, 4)
+        MCB_PHASE_GM(32, 8)
+        MCB_PHASE_GM(32, 4)
+        MCB_PHASE_GM(16, 16)
+        MCB_PHASE_GM(16, 8)
+#undef MCB_PHASE_GM
+        return hipErrorInvalidValue;
+    }
+    const CaseView& c = c0;
     const unsigned gx = (unsigned)(c.geo.nx + PBX - 1) / PBX, gy = (unsigned)(c.geo.ny + PBY - 1) / PBY;
     hipLaunchKernelGGL(k_phase, dim3(gx * gy * (unsigned)c.geo.nz), dim3(PBX, PBY, 1), 0, s, c, gx, gy);
     return hipGetLastError();

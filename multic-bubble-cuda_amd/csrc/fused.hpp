@@ -91,7 +91,8 @@ int tiling_count();
 Tiling tiling_at(int i);
 
 // ---- single case, FP32 populations (bit-exact form)
-hipError_t launch_phase(const CaseView& c, hipStream_t s);
+// Phase pass; in moment form (gm == GM_MOMENTS) it runs on the tiles t.
+hipError_t launch_phase(const CaseView& c, const Tiling& t, hipStream_t s);
 hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStream_t s);
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
 // initTensor + initDist (src/init.cu:21-24, 69-79). g may be null (moment form).

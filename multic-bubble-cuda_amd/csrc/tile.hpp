@@ -154,6 +154,100 @@ __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>
     return sum;
 }
 
+// Moment-form phase pass for the TX x TY column (x0, y0) over planes
+// [zb, ze): the state of each source cell is staged once per plane into a
+// 3-plane LDS ring with a 1-cell halo, as the factors collisionPhase builds
+// from it (w1 phi, w2 phi, phi_norm = sharp_c phi (1 - phi), u, n), and each
+// cell then sums g*_l(x - c_l) = (w_l phi)(1 + c.u/cs^2) + w_l (phi_norm c.n)
+// over l in link order. Every factor is formed by the same operations as in
+// collisionPhase (lbm.cu:319-326), so the sum has the stored populations'
+// bits. Must be called by all TX*TY threads (it contains barriers).
+template <int TX, int TY>
+__device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0, int zb, int ze) {
+    constexpr int NT = TX * TY;
+    constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
+    constexpr int KH = (HN + NT - 1) / NT;
+    enum { W1PHI, W2PHI, PHIN, UX, UY, UZ, NX, NY, NZ, NF };
+    __shared__ float s_m[3][NF][HN];
+
+    const Geo& G = c.geo;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const float w0 = c.phys.w[0], w1 = c.phys.w[1], w2 = c.phys.w[2], sc = c.phys.sharp_c;
+
+    unsigned hoff[KH];
+#pragma unroll
+    for (int q = 0; q < KH; ++q) {
+        const int e = tid + q * NT;
+        unsigned off = 0;
+        if (e < HN) {
+            const int hx = e % HX, hy = e / HX;
+            off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
+        }
+        hoff[q] = off;
+    }
+    auto stage = [&](int z) {
+        const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
+        float (*dst)[HN] = s_m[wrapmod(z, 3)];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN) {
+                const unsigned o = base + hoff[q];
+                const float phi = ld32(c.phi_prev, o);
+                dst[W1PHI][e] = w1 * phi;
+                dst[W2PHI][e] = w2 * phi;
+                dst[PHIN][e] = sc * phi * (1.0f - phi);
+                dst[UX][e] = ld32(c.gm_u[0], o);
+                dst[UY][e] = ld32(c.gm_u[1], o);
+                dst[UZ][e] = ld32(c.gm_u[2], o);
+                dst[NX][e] = ld32(c.gm_n[0], o);
+                dst[NY][e] = ld32(c.gm_n[1], o);
+                dst[NZ][e] = ld32(c.gm_n[2], o);
+            }
+        }
+    };
+
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    const int e0 = (ty + 1) * HX + (tx + 1);
+    stage(zb - 1);
+    stage(zb);
+    for (int z = zb; z < ze; ++z) {
+        // The slot of plane z+1 held plane z-2, last read in iteration z-1
+        // before its closing barrier.
+        stage(z + 1);
+        __syncthreads();
+        if (valid) {
+            const unsigned me = (unsigned)(z * G.nxy + y * G.nx + x);
+            float sum = 0.0f;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (l == 0) {
+                    // phase_post<0> of the cell itself: c_0 = 0.
+                    const float phi = ld32(c.phi_prev, me);
+                    const float phin = sc * phi * (1.0f - phi);
+                    const float geq = w0 * phi * (1.0f + cdot<0>(0.0f, 0.0f, 0.0f) * INV_CSSQ);
+                    sum += geq + w0 * (phin * cdot<0>(0.0f, 0.0f, 0.0f));
+                } else {
+                    const float(*p)[HN] = s_m[wrapmod(z - CZ[l], 3)];
+                    const int e = e0 - CY[l] * HX - CX[l];
+                    float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+                    if constexpr (CX[l] != 0) { ax = p[UX][e]; bx = p[NX][e]; }
+                    if constexpr (CY[l] != 0) { ay = p[UY][e]; by = p[NY][e]; }
+                    if constexpr (CZ[l] != 0) { az = p[UZ][e]; bz = p[NZ][e]; }
+                    const float udotc = cdot<l>(ax, ay, az) * INV_CSSQ;
+                    const float wphi = WC[l] == 1 ? p[W1PHI][e] : p[W2PHI][e];
+                    const float w = WC[l] == 1 ? w1 : w2;
+                    const float geq = wphi * (1.0f + udotc);
+                    sum += geq + w * (p[PHIN][e] * cdot<l>(bx, by, bz));
+                }
+            });
+            st32(c.phi, me, sum);
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- collide
 
 // One cell of the collide pass for a 16-bit form (see lattice_fast.hpp):
@@ -463,6 +557,12 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
     const WorkItem w = work[blockIdx.x];
     const CaseView c = cases[w.c];
     const Geo& G = c.geo;
+    if constexpr (!S::kAnchored) {
+        if (c.gm == GM_MOMENTS) {   // uniform per workgroup
+            phase_tile_gm<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
+            return;
+        }
+    }
     const int x = w.x0 + (int)threadIdx.x, y = w.y0 + (int)threadIdx.y;
     const bool valid = x < G.nx && y < G.ny;
     if constexpr (!S::kAnchored) {
