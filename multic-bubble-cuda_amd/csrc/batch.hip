@@ -41,8 +41,10 @@ std::vector<WorkItem> xcd_permute(const std::vector<WorkItem>& in) {
 
 size_t case_bytes(const mcb_params& p) {
     const size_t P = (size_t)plane_of(p);
-    size_t b = 4 * 19 * P * pop_bytes(p.storage);
-    b += (p.storage == MCB_STORE_FP32 ? 1 : 4) * P * sizeof(float);
+    const bool moments = p.tau_g == 1.0f;
+    size_t b = (moments ? 2 : 4) * 19 * P * pop_bytes(p.storage);          // f (and g) A/B
+    b += (moments || p.storage != MCB_STORE_FP32 ? 2 : 1) * P * sizeof(float);   // phi
+    if (moments) b += 6 * P * sizeof(float);                                // u, n
     return b;
 }
 
@@ -57,8 +59,6 @@ CaseView CaseState::view(int q) const {
     v.gdst = g[q ^ 1];
     v.phi = phi[q];
     v.phi_prev = phi[q ^ 1];
-    v.rho_prev = rho_anchor[q ^ 1];
-    v.rho_next = rho_anchor[q];
     v.mslot = q;
     v.mass = mass;
     v.partial = partial;
@@ -245,7 +245,9 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     const size_t pops = 19 * (size_t)P * pop_bytes(p.storage);
     // FP32 at tau_g = 1 keeps g in moment form (phi, u, n of the last
     // collision) instead of two sets of 19 planes.
-    const bool moments = p.storage == MCB_STORE_FP32 && c->mode == COLLIDE_STEP;
+    // At tau_g = 1 g is kept in moment form (phi, u, n of the last collision,
+    // FP32) instead of two sets of 19 planes, for every storage form.
+    const bool moments = c->mode == COLLIDE_STEP;
     for (int b = 0; b < 2; ++b) {
         if ((st = alloc(pops, &c->f[b])) != MCB_OK) return st;
         if (!moments && (st = alloc(pops, &c->g[b])) != MCB_OK) return st;
@@ -253,22 +255,19 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     void* q = nullptr;
     if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
     c->phi[0] = c->phi[1] = static_cast<float*>(q);
-    if (moments) {
+    if (moments || p.storage != MCB_STORE_FP32) {
+        // phi[1] holds the state's phi (moment form) or the g anchor (16 bits).
         if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
         c->phi[1] = static_cast<float*>(q);
+    }
+    if (moments) {
         for (int k = 0; k < 6; ++k) {
             if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
             (k < 3 ? c->gmu[k] : c->gmn[k - 3]) = static_cast<float*>(q);
         }
         c->gm = GM_INITIAL;
-    }
-    if (p.storage != MCB_STORE_FP32) {
-        if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
-        c->phi[1] = static_cast<float*>(q);
-        for (int b = 0; b < 2; ++b) {
-            if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
-            c->rho_anchor[b] = static_cast<float*>(q);
-        }
+    } else if (p.storage != MCB_STORE_FP32) {
+        // 16-bit phase populations: per-step FP64 mass-drift correction.
         if ((st = alloc(sizeof(MassCell), &q)) != MCB_OK) return st;
         c->mass = static_cast<MassCell*>(q);
         double target = 0.0;
@@ -276,8 +275,8 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
             for (long long i = 0; i < c->geo.n; ++i) target += (double)phi_host[i];
         if ((st = set_mass_target(c, target)) != MCB_OK) return st;
     }
-    // Initial state: the current populations are in buffer 0, their anchors
-    // (16-bit forms) in phi[1] and rho_anchor[1].
+    // Initial state: the current populations are in buffer 0, the initial
+    // phi (state or g anchor) in phi[1].
     if (phi_host) {
         B_HIP(hipMemcpyAsync(c->phi[1], phi_host, (size_t)c->geo.n * sizeof(float), hipMemcpyHostToDevice, stream_));
     } else {
@@ -288,11 +287,10 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
         }
     }
     if (p.storage == MCB_STORE_FP32) {
-        // (moment form: g[0] is null and phi[1] keeps phi0 as the state)
         B_HIP(launch_init_pull(c->phi[1], static_cast<float*>(c->f[0]), static_cast<float*>(c->g[0]), c->geo, c->phys,
                                stream_));
     } else {
-        B_HIP(launch_init16(c->phi[1], c->f[0], c->g[0], c->rho_anchor[1], c->geo, c->phys, p.storage, stream_));
+        B_HIP(launch_init16(c->phi[1], c->f[0], c->g[0], c->geo, c->phys, p.storage, stream_));
     }
     c->cur = 0;
     c->parity_offset = parity_;
@@ -559,8 +557,8 @@ mcb_status Batch::read(int id, int field, float* host, size_t n) {
         else if (c->storage == MCB_STORE_FP32)
             e = launch_link_shift(static_cast<const float*>(isf ? c->f[c->cur] : c->g[c->cur]), tmp, c->geo, +1, stream_);
         else
-            e = launch_decode16(isf ? c->f[c->cur] : c->g[c->cur], isf ? c->rho_anchor[c->cur ^ 1] : c->phi[c->cur ^ 1],
-                                tmp, c->geo, c->phys, c->storage, stream_);
+            e = launch_decode16(isf ? c->f[c->cur] : c->g[c->cur], isf ? nullptr : c->phi[c->cur ^ 1], tmp, c->geo,
+                                c->phys, c->storage, stream_);
         for (int l = 0; l < 19 && e == hipSuccess; ++l)
             e = hipMemcpyAsync(host + l * cells, tmp + l * P, cells * sizeof(float), hipMemcpyDeviceToHost, stream_);
         if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -594,14 +592,23 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
         const bool isf = field == MCB_FIELD_F;
         if (!isf && c->gm != GM_POPULATIONS) {
             // Arbitrary g populations have no moment form: the case keeps g as
-            // populations from here on.
+            // populations from here on (16 bits: anchored to the state's phi,
+            // with the mass-drift correction).
             B_HIP(hipStreamSynchronize(stream_));
-            const size_t pops = 19 * P * sizeof(float);
+            const size_t pops = 19 * P * pop_bytes(c->storage);
             for (int b = 0; b < 2; ++b) {
                 mcb_status st = dev_alloc(pops, &c->g[b]);
                 if (st != MCB_OK) return st;
                 c->allocs.emplace_back(c->g[b], pops);
                 c->bytes += pops;
+            }
+            if (c->storage != MCB_STORE_FP32) {
+                void* q = nullptr;
+                mcb_status st = dev_alloc(sizeof(MassCell), &q);
+                if (st != MCB_OK) return st;
+                c->allocs.emplace_back(q, sizeof(MassCell));
+                c->mass = static_cast<MassCell*>(q);
+                if ((st = build_own_table(c)) != MCB_OK) return st;   // partial sums
             }
             c->gm = GM_POPULATIONS;
             dirty_ = true;
@@ -615,8 +622,8 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
             if (c->storage == MCB_STORE_FP32)
                 e = launch_link_shift(tmp, static_cast<float*>(isf ? c->f[c->cur] : c->g[c->cur]), c->geo, -1, stream_);
             else
-                e = launch_encode16(tmp, isf ? c->rho_anchor[c->cur ^ 1] : c->phi[c->cur ^ 1],
-                                    isf ? c->f[c->cur] : c->g[c->cur], c->geo, c->phys, c->storage, stream_);
+                e = launch_encode16(tmp, isf ? nullptr : c->phi[c->cur ^ 1], isf ? c->f[c->cur] : c->g[c->cur], c->geo,
+                                    c->phys, c->storage, stream_);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(stream_);
         (void)hipFree(tmp);
@@ -630,7 +637,7 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
         return MCB_OK;
     }
     if (field != MCB_FIELD_PHI) return fail(MCB_ERR_INVALID_ARG, "only PHI, F and G can be written");
-    if (c->storage != MCB_STORE_FP32)
+    if (c->storage != MCB_STORE_FP32 && c->gm == GM_POPULATIONS)
         return fail(MCB_ERR_UNSUPPORTED, "with 16-bit populations phi anchors the g populations; write G instead");
     if (n != cells) return fail(MCB_ERR_INVALID_ARG, "field buffer must hold nx*ny*nz floats");
     // In moment form phi of the state is phi[cur ^ 1]; like a write with

@@ -33,7 +33,6 @@ struct CaseState {
     void* f[2] = {nullptr, nullptr};
     void* g[2] = {nullptr, nullptr};
     float* phi[2] = {nullptr, nullptr};
-    float* rho_anchor[2] = {nullptr, nullptr};
     float* phi_last = nullptr;      // written by the most recent phase pass
     int gm = GM_POPULATIONS;        // FP32 at tau_g = 1: g in moment form (GMode)
     float* gmu[3] = {nullptr, nullptr, nullptr};
@@ -62,8 +61,8 @@ struct CaseState {
 // Host-side constants of a parameter set (host.cpp).
 Phys host_phys(const mcb_params& p);
 
-// Device bytes of a case with these parameters (populations, phase and
-// anchor fields).
+// Device bytes of a case with these parameters (populations, phase field(s)
+// and, at tau_g = 1, the moment form of g).
 size_t case_bytes(const mcb_params& p);
 
 class Batch {

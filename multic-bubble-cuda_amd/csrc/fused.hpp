@@ -57,10 +57,8 @@ struct CaseView {
     const void* gsrc;
     void* gdst;
     float* phi;             // phi of this step: written by the phase pass
-    const float* phi_prev;  // 16-bit storage: phi of the previous step (anchor of g);
+    const float* phi_prev;  // 16-bit g populations: phi of the previous step (anchor of g);
                             // moment form: phi of the previous step's collision
-    const float* rho_prev;  // 16-bit storage: rho of the previous step (anchor of f)
-    float* rho_next;        // 16-bit storage: rho of this step
     // Moment form of g (FP32, tau_g = 1): g*_l(y) is collisionPhase of
     // (phi, u, n) at y, so the state keeps u and n (gm_u, gm_n) instead of
     // 19 g planes and the phase pass recomputes g*_l(x - c_l).
@@ -112,16 +110,16 @@ hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, uns
 
 // ---- 16-bit populations (a single case runs as a batch of one)
 bool tiling_supported16(const Tiling& t);
-// 16-bit initial state: g*_l(y) = w_l phi(y + c_l) against the anchor phi(y),
-// f*_l = w_l against the anchor rho = 1 (offset 0).
-hipError_t launch_init16(const float* phi, void* f, void* g, float* rho_anchor, const Geo& geo, const Phys& p,
-                         int storage, hipStream_t s);
+// 16-bit initial state: f*_l = w_l (offset 0); g*_l(y) = w_l phi(y + c_l)
+// against the anchor phi(y) when g is stored (g may be null: moment form).
+hipError_t launch_init16(const float* phi, void* f, void* g, const Geo& geo, const Phys& p, int storage,
+                         hipStream_t s);
 // Decode 16-bit pull-form populations to FP32 reference layout:
-// dst_l(x) = w_l anchor(x - c_l) + t_l(x - c_l) / K_l.
+// dst_l(x) = w_l anchor(x - c_l) + t_l(x - c_l) / K_l (anchor null: 1).
 hipError_t launch_decode16(const void* src, const float* anchor, float* dst, const Geo& geo, const Phys& p,
                            int storage, hipStream_t s);
 // Encode FP32 reference-layout populations to 16-bit pull form against the
-// anchor field: t_l(x) = (src_l(x + c_l) - w_l anchor(x)) K_l.
+// anchor field: t_l(x) = (src_l(x + c_l) - w_l anchor(x)) K_l (anchor null: 1).
 hipError_t launch_encode16(const float* src, const float* anchor, void* dst, const Geo& geo, const Phys& p,
                            int storage, hipStream_t s);
 // field[i] += delta for i < n (uniform phase correction of the drift).

@@ -28,11 +28,11 @@ struct Offs {
     }
 };
 
-// Initial pull-form state: f*_l = w_l * 1 with anchor rho = 1 (offset 0);
-// g*_l(y) = w_l * phi(y + c_l) with anchor phi(y).
+// Initial pull-form state: f*_l = w_l (offset 0); g*_l(y) = w_l * phi(y + c_l)
+// with anchor phi(y) when g is stored as populations (g null: moment form).
 template <class S>
-__global__ __launch_bounds__(PBX * PBY) void k_init16(const float* __restrict__ phi, void* f, void* g,
-                                                      float* __restrict__ rho_anchor, Geo G, Phys p) {
+__global__ __launch_bounds__(PBX * PBY) void k_init16(const float* __restrict__ phi, void* f, void* g, Geo G,
+                                                      Phys p) {
     const int x = blockIdx.x * PBX + threadIdx.x;
     const int y = blockIdx.y * PBY + threadIdx.y;
     const int z = blockIdx.z;
@@ -40,12 +40,12 @@ __global__ __launch_bounds__(PBX * PBY) void k_init16(const float* __restrict__ 
     const Offs o(x, y, z, G);
     const unsigned me = o.me();
     const PopPlanes<S> fp(f, (unsigned)G.plane), gp(g, (unsigned)G.plane);
+    for_links([&](auto lc) { fp.template st<decltype(lc)::value>(me, 0.0f); });
+    if (!g) return;
     const float ph = phi[me];
-    rho_anchor[me] = 1.0f;
     float rg = 0.0f;
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        fp.template st<l>(me, 0.0f);
         if constexpr (l != 0) {
             const float s = offset_of_pop<l>(p, ph, p.w[WC[l]] * phi[o.template nb<l, 1>()]);
             rg += (s - gp.template st<l>(me, s)) * off_unscale(WC[l]);
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_decode16(const void* src, const f
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const unsigned s = o.template nb<l, -1>();
-        dst[(size_t)l * G.plane + o.me()] = pop_from_offset<l>(p, anchor[s], sp.template ld<l>(s));
+        dst[(size_t)l * G.plane + o.me()] = pop_from_offset<l>(p, anchor ? anchor[s] : 1.0f, sp.template ld<l>(s));
     });
 }
 
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_encode16(const float* __restrict_
     const Offs o(x, y, z, G);
     const unsigned me = o.me();
     const PopPlanes<S> dp(dst, (unsigned)G.plane);
-    const float a = anchor[me];
+    const float a = anchor ? anchor[me] : 1.0f;
     float r = 0.0f;
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
@@ -152,12 +152,12 @@ hipError_t launch_collide_batch16(const CaseView* cases, const WorkItem* work, u
     return batch16(cases, work, n_work, t, mode, false, storage, s);
 }
 
-hipError_t launch_init16(const float* phi, void* f, void* g, float* rho_anchor, const Geo& geo, const Phys& p,
-                         int storage, hipStream_t s) {
+hipError_t launch_init16(const float* phi, void* f, void* g, const Geo& geo, const Phys& p, int storage,
+                         hipStream_t s) {
     if (storage == MCB_STORE_FP16)
-        hipLaunchKernelGGL(k_init16<StoreF16>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, rho_anchor, geo, p);
+        hipLaunchKernelGGL(k_init16<StoreF16>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
     else if (storage == MCB_STORE_BF16)
-        hipLaunchKernelGGL(k_init16<StoreBF16>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, rho_anchor, geo, p);
+        hipLaunchKernelGGL(k_init16<StoreBF16>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -4,13 +4,15 @@
 // Storage forms (template parameter S):
 //   StoreF32  : populations stored as FP32, read and written as computed.
 //               This form reproduces the FP32 transcription bit for bit.
-//   StoreF16 / StoreBF16 : populations stored in 16 bits as offsets from
-//               w_l times an FP32 anchor of their source cell:
-//                 f*_l(y) = w_l rho(y) + t_l(y) / K_l,  g*_l(y) = w_l phi(y) + t_l(y) / K_l,
-//               where rho(y), phi(y) are the density and phase the source cell
-//               had when it collided (kept as FP32 fields). The offsets are
-//               small (velocity and sharpening terms), so 16 bits keep about
-//               1e-5 of w_l. K_l is a power of two near 2^12 / w_l: an
+//   StoreF16 / StoreBF16 : populations stored in 16 bits as offsets:
+//                 f*_l(y) = w_l + t_l(y) / K_l              (from the lattice weight)
+//                 g*_l(y) = w_l phi(y) + t_l(y) / K_l       (tau_g != 1)
+//               where phi(y) is the phase the source cell had when it
+//               collided (an FP32 field). At tau_g = 1 g is not stored as
+//               populations at all but in the FP32 moment form (phi, u, n;
+//               see phase_tile_gm). The offsets are small (density, velocity
+//               and sharpening terms), so 16 bits keep about 1e-5 of w_l
+//               (FP16). K_l is a power of two near 2^12 / w_l: an
 //               exact scale, so encoding and decoding carry no systematic
 //               bias (a rounded 1/w_l would scale every population by the
 //               same w_l * fl(1/w_l) != 1 and drift the mass each step), and
@@ -131,19 +133,17 @@ template <class S>
 __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const unsigned (&xs)[3],
                                            const unsigned (&ys)[3], const unsigned (&zs)[3]) {
     float sum = 0.0f;
-    if constexpr (!S::kAnchored) {
-        if (c.gm == GM_MOMENTS) {
-            for_links([&](auto lc) {
-                constexpr int l = decltype(lc)::value;
-                sum += g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
-            });
-            return sum;
-        }
-        if (c.gm == GM_INITIAL) {
-            const float p0 = ld32(c.phi_prev, zs[1] + ys[1] + xs[1]);
-            for_links([&](auto lc) { sum += c.phys.w[WC[decltype(lc)::value]] * p0; });
-            return sum;
-        }
+    if (c.gm == GM_MOMENTS) {
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            sum += g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+        });
+        return sum;
+    }
+    if (c.gm == GM_INITIAL) {
+        const float p0 = ld32(c.phi_prev, zs[1] + ys[1] + xs[1]);
+        for_links([&](auto lc) { sum += c.phys.w[WC[decltype(lc)::value]] * p0; });
+        return sum;
     }
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
@@ -251,9 +251,10 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
 // ---------------------------------------------------------------- collide
 
 // One cell of the collide pass for a 16-bit form (see lattice_fast.hpp):
-// surface-tension force, moments, and the post-collision offsets of f and g
-// written against this cell's rho and phi. Links 1..18 go first; their
-// rounding residuals (in population units) then go into the rest link.
+// surface-tension force, moments, and the post-collision offsets of f,
+// written against w_l, and of g, written against w_l phi of this cell (or, in
+// moment form, this cell's u and n). Links 1..18 go first; their rounding
+// residuals (in population units) then go into the rest link.
 template <int MODE, class S, class NBN>
 __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k, const PopPlanes<S>& fsrc,
                                        const PopPlanes<S>& gsrc, const PopPlanes<S>& fdst,
@@ -265,8 +266,7 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
     float fv[Q];
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
-        fv[l] = pop_from_offset<l>(c.phys, ld32(c.rho_prev, src), fsrc.template ld<l>(src));
+        fv[l] = pop_from_offset<l>(c.phys, 1.0f, fsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]));
     });
     const fast::Fluid fl = fast::fluid(fv, k, Fx, Fy, Fz);
     if constexpr (MODE == COLLIDE_MACROS) {
@@ -281,7 +281,37 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
         st32(c.ny, me, ncy);
         st32(c.nz, me, ncz);
     } else {
-        st32(c.rho_next, me, fl.rho);
+        auto put = [&](auto lc, const PopPlanes<S>& dst, float off, float& r) {
+            constexpr int l = decltype(lc)::value;
+            const float t = off * off_scale(WC[l]);
+            r += (t - dst.template st<l>(me, t)) * off_unscale(WC[l]);
+        };
+        // f*_l - w_l = (f*_l - w_l rho) + w_l (rho - 1)
+        const float drho = fl.rho - 1.0f;
+        const float dr1 = k.w1 * drho, dr2 = k.w2 * drho;
+        float rf = 0.0f;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (l % 2 == 1) {
+                float op, om;
+                fast::fluid_pair<l>(fl, k, Fx, Fy, Fz, op, om);
+                const float dr = WC[l] == 1 ? dr1 : dr2;
+                put(LinkC<l>{}, fdst, op + dr, rf);
+                put(LinkC<l + 1>{}, fdst, om + dr, rf);
+            }
+        });
+        fdst.template st<0>(me, ((fast::fluid_rest(fl, k) + k.w0 * drho) + rf) * off_scale(0));
+
+        if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
+            // Moment form of g (tau_g = 1): what collisionPhase needs.
+            st32(c.gm_u[0], me, fl.ux);
+            st32(c.gm_u[1], me, fl.uy);
+            st32(c.gm_u[2], me, fl.uz);
+            st32(c.gm_n[0], me, ncx);
+            st32(c.gm_n[1], me, ncy);
+            st32(c.gm_n[2], me, ncz);
+            return;
+        }
         const float phin = k.sharp_c * phic * (1.0f - phic);
         // g offset of link l after the phase collision, from its odd part.
         auto g_off = [&](auto lc, float odd) {
@@ -297,25 +327,15 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
                 return k.one_m_omega_g * pre + k.omega_g * odd;
             }
         };
-        float rf = 0.0f, rg = 0.0f;
-        auto put = [&](auto lc, const PopPlanes<S>& dst, float off, float& r) {
-            constexpr int l = decltype(lc)::value;
-            const float t = off * off_scale(WC[l]);
-            r += (t - dst.template st<l>(me, t)) * off_unscale(WC[l]);
-        };
+        float rg = 0.0f;
         for_links([&](auto lc) {
             constexpr int l = decltype(lc)::value;
             if constexpr (l % 2 == 1) {
-                float op, om;
-                fast::fluid_pair<l>(fl, k, Fx, Fy, Fz, op, om);
-                put(LinkC<l>{}, fdst, op, rf);
-                put(LinkC<l + 1>{}, fdst, om, rf);
                 const float odd = fast::phase_odd<l>(k, phic, phin, fl.ux, fl.uy, fl.uz, ncx, ncy, ncz);
                 put(LinkC<l>{}, gdst, g_off(LinkC<l>{}, odd), rg);
                 put(LinkC<l + 1>{}, gdst, g_off(LinkC<l + 1>{}, -odd), rg);
             }
         });
-        fdst.template st<0>(me, (fast::fluid_rest(fl, k) + rf) * off_scale(0));
         gdst.template st<0>(me, (g_off(LinkC<0>{}, 0.0f) + rg) * off_scale(0));
     }
 }
@@ -557,17 +577,15 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
     const WorkItem w = work[blockIdx.x];
     const CaseView c = cases[w.c];
     const Geo& G = c.geo;
-    if constexpr (!S::kAnchored) {
-        if (c.gm == GM_MOMENTS) {   // uniform per workgroup
-            phase_tile_gm<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
-            return;
-        }
+    if (c.gm == GM_MOMENTS) {   // uniform per workgroup
+        phase_tile_gm<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
+        return;
     }
+    // 16-bit phase populations carry the mass bookkeeping (a barrier inside).
+    const bool mass = S::kAnchored && c.gm == GM_POPULATIONS;
     const int x = w.x0 + (int)threadIdx.x, y = w.y0 + (int)threadIdx.y;
     const bool valid = x < G.nx && y < G.ny;
-    if constexpr (!S::kAnchored) {
-        if (!valid) return;
-    }
+    if (!mass && !valid) return;
     const unsigned xs[3] = {valid ? (unsigned)shift<-1>(x, G.nx) : 0u, (unsigned)x,
                             valid ? (unsigned)shift<1>(x, G.nx) : 0u};
     const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
@@ -576,20 +594,20 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
     const int ze = min(w.z0 + c.zc, G.nz);
     double part = 0.0;
     float delta = 0.0f;
-    if constexpr (S::kAnchored) delta = (float)c.mass->delta[c.mslot];
+    if (mass) delta = (float)c.mass->delta[c.mslot];
     if (valid) {
         for (int z = w.z0; z < ze; ++z) {
             const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
                                     (unsigned)(shift<1>(z, G.nz) * G.nxy)};
             float v = phase_sum<S>(c, gp, xs, ys, zs);
-            if constexpr (S::kAnchored) {
+            if (mass) {
                 v += delta;
                 part += (double)v;
             }
             st32(c.phi, zs[1] + ys[1] + xs[1], v);
         }
     }
-    if constexpr (S::kAnchored) mass_partial<TX, TY>(c, w, part);
+    if (mass) mass_partial<TX, TY>(c, w, part);
 }
 
 template <int TX, int TY, int MODE, class S>

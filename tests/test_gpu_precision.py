@@ -18,15 +18,26 @@ pytestmark = pytest.mark.gpu
 
 CS2 = 1.0 / 3.0
 DP_TOL = 0.01          # |dp / dp_fp64 - 1|
-RATIO, FLOOR = 4.0, 2e-3
+# Per-field bound: RATIO x (FP32's distance from FP64) + FLOOR[field]. BF16
+# keeps 8 significand bits to FP16's 11 and f is stored as offsets from the
+# lattice weights, so in smooth, quiet flows (tau_f = 1, spurious currents of
+# ~1e-4) its u carries several percent of error where FP32 is far closer to
+# FP64; its bounds state that. Both forms must meet the Laplace-jump bar.
+RATIO = {"fp16": 4.0, "bf16": 16.0}
+FLOOR = {"fp16": {"phi": 2e-3, "rho": 2e-3, "u": 2e-3},
+         "bf16": {"phi": 1e-2, "rho": 1e-2, "u": 0.15}}
 SUM_PHI_TOL = 1e-6     # relative change of sum(phi) over the run
 
-# (n, R, sigma, tau_f, tau_g, steps): two Laplace cases (near equilibrium) and
-# the 64^3 plain-method case.
+# (n, R, sigma, tau_f, tau_g, steps): Laplace cases (near equilibrium) and the
+# 64^3 plain-method case, with g stored as 16-bit populations (tau_g != 1)
+# and in FP32 moment form (tau_g = 1).
 CASES = [
     (48, 10.0, 0.1, 1.0, 0.8, 800),
     (48, 10.0, 0.01, 1.0, 0.8, 800),
     (64, 10.0, 0.1, 0.55, 0.8, 1000),
+    (48, 10.0, 0.1, 1.0, 1.0, 800),
+    (48, 10.0, 0.01, 0.7, 1.0, 800),
+    (64, 10.0, 0.1, 0.52, 1.0, 1000),
 ]
 
 
@@ -70,7 +81,7 @@ def fp64_and_fp32(mcb, case):
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c[0]}-sigma{c[2]}")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c[0]}-sigma{c[2]}-taug{c[4]}")
 def test_16bit_storage_against_fp64(gpu_lib, storage, case):
     mcb = gpu_lib
     n, R, sigma, tf, tg, steps = case
@@ -94,7 +105,7 @@ def test_16bit_storage_against_fp64(gpu_lib, storage, case):
     assert abs(dp / dp64 - 1) <= DP_TOL, f"Laplace jump {dp} vs FP64 {dp64}"
     assert abs(s1 - s0) / s0 <= SUM_PHI_TOL
     for k in e:
-        assert e[k] <= RATIO * e32[k] + FLOOR, f"{k}: {e[k]:.3e} vs FP32's {e32[k]:.3e}"
+        assert e[k] <= RATIO[storage] * e32[k] + FLOOR[storage][k], f"{k}: {e[k]:.3e} vs FP32's {e32[k]:.3e}"
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])

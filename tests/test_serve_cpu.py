@@ -75,10 +75,16 @@ def test_serve_run_rejects_bad_arguments(mcb):
 
 
 def test_case_bytes(mcb):
-    p = mcb.make_params(64, 64, 64)
     n = 64 ** 3
+    # tau_g != 1: f and g populations (A/B); 16 bits add a second phi (g anchor)
+    p = mcb.make_params(64, 64, 64, tau_g=0.8)
     assert mcb.case_bytes(p) == 4 * 19 * n * 4 + n * 4
     p.storage = mcb.STORE_FP16
-    assert mcb.case_bytes(p) == 4 * 19 * n * 2 + 4 * n * 4
+    assert mcb.case_bytes(p) == 4 * 19 * n * 2 + 2 * n * 4
+    # tau_g = 1: f populations and g in moment form (two phi, u, n)
+    p = mcb.make_params(64, 64, 64, tau_g=1.0)
+    assert mcb.case_bytes(p) == 2 * 19 * n * 4 + 8 * n * 4
+    p.storage = mcb.STORE_FP16
+    assert mcb.case_bytes(p) == 2 * 19 * n * 2 + 8 * n * 4
     p.tau_f = 0.4
     assert mcb.case_bytes(p) == 0   # invalid parameters
