@@ -33,12 +33,20 @@ template <int C> __device__ __forceinline__ int shift(int a, int n) {
 // byte offset is formed in 32 bits, so the access compiles to a global
 // load/store with the base in scalar registers and one 32-bit vector offset
 // (no per-lane 64-bit address arithmetic). Needs elem * 4 < 2^32, i.e. planes
-// of fewer than 2^30 cells (enforced by parameter validation).
+// of fewer than 2^30 cells (enforced by parameter validation). The pointer is
+// taken as a global-memory (address space 1) pointer: fields reached through
+// a CaseView read from device memory would otherwise be generic pointers and
+// compile to flat accesses, which share the LDS counter and serialise the
+// LDS staging against the gathers.
+typedef const float __attribute__((address_space(1)))* gconst_fp;
+typedef float __attribute__((address_space(1)))* g_fp;
 __device__ __forceinline__ float ld32(const float* __restrict__ base, unsigned elem) {
-    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (elem << 2));
+    const gconst_fp p = (gconst_fp)base;
+    return *(gconst_fp)((const char __attribute__((address_space(1)))*)p + (elem << 2));
 }
 __device__ __forceinline__ void st32(float* __restrict__ base, unsigned elem, float v) {
-    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (elem << 2)) = v;
+    const g_fp p = (g_fp)base;
+    *(g_fp)((char __attribute__((address_space(1)))*)p + (elem << 2)) = v;
 }
 
 // The 19 population planes of one buffer behind a single buffer resource:
@@ -55,6 +63,10 @@ constexpr int kBufferConfigWord = 0x00020000;   // raw 32-bit buffer, gfx9 famil
 // phase pass at 256^3, because fewer dirty lines are evicted during it.
 #ifndef MCB_STORE_AUX
 #define MCB_STORE_AUX 2
+#endif
+// The same bits for the 2-byte stores of the 16-bit forms.
+#ifndef MCB_STORE16_AUX
+#define MCB_STORE16_AUX 2
 #endif
 struct Planes {
     __amdgpu_buffer_rsrc_t r;

@@ -109,7 +109,7 @@ template <class S> struct PopPlanes {
     // Stores the 16-bit form of v and returns the value it decodes to.
     template <int L> __device__ __forceinline__ float st(unsigned elem, float v) const {
         const unsigned short b = S::enc(v);
-        __builtin_amdgcn_raw_buffer_store_b16(b, r, elem << 1, L * plane_bytes, MCB_STORE_AUX);
+        __builtin_amdgcn_raw_buffer_store_b16(b, r, elem << 1, L * plane_bytes, MCB_STORE16_AUX);
         return S::dec(b);
     }
 };
