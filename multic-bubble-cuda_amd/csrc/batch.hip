@@ -44,7 +44,7 @@ size_t case_bytes(const mcb_params& p) {
     const bool moments = p.tau_g == 1.0f;
     size_t b = (moments ? 2 : 4) * 19 * P * pop_bytes(p.storage);          // f (and g) A/B
     b += (moments || p.storage != MCB_STORE_FP32 ? 2 : 1) * P * sizeof(float);   // phi
-    if (moments) b += 6 * P * sizeof(float);                                // u, n
+    if (moments) b += (p.storage == MCB_STORE_FP32 ? 6 : 3) * P * sizeof(float);   // u, n or B
     return b;
 }
 
@@ -261,7 +261,10 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
         c->phi[1] = static_cast<float*>(q);
     }
     if (moments) {
-        for (int k = 0; k < 6; ++k) {
+        // FP32: u and n (collisionPhase's inputs, bit-exact); 16 bits: the
+        // vector B = 3 phi u + phi_n n (phase_tile_ab).
+        const int nfields = p.storage == MCB_STORE_FP32 ? 6 : 3;
+        for (int k = 0; k < nfields; ++k) {
             if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
             (k < 3 ? c->gmu[k] : c->gmn[k - 3]) = static_cast<float*>(q);
         }
@@ -553,7 +556,8 @@ mcb_status Batch::read(int id, int field, float* host, size_t n) {
         B_HIP(hipMalloc(reinterpret_cast<void**>(&tmp), 19 * P * sizeof(float)));
         hipError_t e;
         if (!isf && c->gm != GM_POPULATIONS)
-            e = launch_gm_to_ref(c->view(c->cur), tmp, stream_);
+            e = c->storage == MCB_STORE_FP32 ? launch_gm_to_ref(c->view(c->cur), tmp, stream_)
+                                             : launch_ab_to_ref(c->view(c->cur), tmp, stream_);
         else if (c->storage == MCB_STORE_FP32)
             e = launch_link_shift(static_cast<const float*>(isf ? c->f[c->cur] : c->g[c->cur]), tmp, c->geo, +1, stream_);
         else

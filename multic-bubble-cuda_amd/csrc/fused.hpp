@@ -95,8 +95,10 @@ hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStrea
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
 // initTensor + initDist (src/init.cu:21-24, 69-79). g may be null (moment form).
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s);
-// Moment form of g -> reference layout (post-stream g_l(x), 19 planes).
+// Moment form of g -> reference layout (post-stream g_l(x), 19 planes):
+// FP32 (phi, u, n) form, and the (phi, B) form of the 16-bit storage forms.
 hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s);
+hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->
 // reference layout; sign = -1: reference layout -> pull form).
 hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int sign, hipStream_t s);

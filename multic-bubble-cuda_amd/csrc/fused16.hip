@@ -95,6 +95,33 @@ __global__ __launch_bounds__(PBX * PBY) void k_encode16(const float* __restrict_
     dp.template st<0>(me, offset_of_pop<0>(p, a, src[me] + r));
 }
 
+// (phi, B) moment form -> reference layout: dst_l(x) = w_l (phi + c_l.B)(x - c_l),
+// or w_l phi0(x) before the first step.
+__global__ __launch_bounds__(PBX * PBY) void k_ab_to_ref(CaseView c, float* __restrict__ dst) {
+    const Geo& G = c.geo;
+    const int x = blockIdx.x * PBX + threadIdx.x;
+    const int y = blockIdx.y * PBY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= G.nx || y >= G.ny) return;
+    const Offs o(x, y, z, G);
+    const unsigned me = o.me();
+    for_links([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        float v;
+        if (c.gm == GM_INITIAL) {
+            v = c.phys.w[WC[l]] * ld32(c.phi_prev, me);
+        } else {
+            const unsigned s = o.template nb<l, -1>();
+            float a = ld32(c.phi_prev, s);
+            if constexpr (CX[l] != 0) a = CX[l] > 0 ? a + ld32(c.gm_u[0], s) : a - ld32(c.gm_u[0], s);
+            if constexpr (CY[l] != 0) a = CY[l] > 0 ? a + ld32(c.gm_u[1], s) : a - ld32(c.gm_u[1], s);
+            if constexpr (CZ[l] != 0) a = CZ[l] > 0 ? a + ld32(c.gm_u[2], s) : a - ld32(c.gm_u[2], s);
+            v = c.phys.w[WC[l]] * a;
+        }
+        dst[(size_t)l * G.plane + me] = v;
+    });
+}
+
 template <int TX, int TY, class S>
 hipError_t batch_t(const CaseView* cases, const WorkItem* work, unsigned n, int mode, bool phase, hipStream_t s) {
     dim3 grid(n, 1, 1), block(TX, TY, 1);
@@ -160,6 +187,12 @@ hipError_t launch_init16(const float* phi, void* f, void* g, const Geo& geo, con
         hipLaunchKernelGGL(k_init16<StoreBF16>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s) {
+    if (c.gm == GM_POPULATIONS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ab_to_ref, cell_grid(c.geo), dim3(PBX, PBY, 1), 0, s, c, dst);
     return hipGetLastError();
 }
 

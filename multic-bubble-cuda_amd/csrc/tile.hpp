@@ -248,6 +248,78 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
     }
 }
 
+// Moment form of the 16-bit storage forms (not bit-exact, so the algebra is
+// free): collisionPhase's g*_l(y) = w_l phi (1 + 3 c_l.u) + w_l phi_n c_l.n is
+// w_l (phi + c_l.B) with B = 3 phi u + phi_n n, so the state keeps phi and
+// the vector B (in gm_u) and the phase pass sums
+//   phi(x) = w0 phi(x) + w1 sum_faces (phi + c.B)(x - c) + w2 sum_edges (...).
+// 4 fields per source cell instead of 7, and a handful of adds per link.
+template <int TX, int TY>
+__device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0, int zb, int ze) {
+    constexpr int NT = TX * TY;
+    constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
+    constexpr int KH = (HN + NT - 1) / NT;
+    __shared__ float s_a[3][4][HN];   // phi, Bx, By, Bz
+
+    const Geo& G = c.geo;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const float w0 = c.phys.w[0], w1 = c.phys.w[1], w2 = c.phys.w[2];
+    unsigned hoff[KH];
+#pragma unroll
+    for (int q = 0; q < KH; ++q) {
+        const int e = tid + q * NT;
+        unsigned off = 0;
+        if (e < HN) {
+            const int hx = e % HX, hy = e / HX;
+            off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
+        }
+        hoff[q] = off;
+    }
+    auto stage = [&](int z) {
+        const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
+        float (*dst)[HN] = s_a[wrapmod(z, 3)];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN) {
+                const unsigned o = base + hoff[q];
+                dst[0][e] = ld32(c.phi_prev, o);
+                dst[1][e] = ld32(c.gm_u[0], o);
+                dst[2][e] = ld32(c.gm_u[1], o);
+                dst[3][e] = ld32(c.gm_u[2], o);
+            }
+        }
+    };
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    const int e0 = (ty + 1) * HX + (tx + 1);
+    stage(zb - 1);
+    stage(zb);
+    for (int z = zb; z < ze; ++z) {
+        stage(z + 1);
+        __syncthreads();
+        if (valid) {
+            float faces = 0.0f, edges = 0.0f;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (l != 0) {
+                    const float(*p)[HN] = s_a[wrapmod(z - CZ[l], 3)];
+                    const int e = e0 - CY[l] * HX - CX[l];
+                    float v = p[0][e];
+                    if constexpr (CX[l] != 0) v = CX[l] > 0 ? v + p[1][e] : v - p[1][e];
+                    if constexpr (CY[l] != 0) v = CY[l] > 0 ? v + p[2][e] : v - p[2][e];
+                    if constexpr (CZ[l] != 0) v = CZ[l] > 0 ? v + p[3][e] : v - p[3][e];
+                    if constexpr (WC[l] == 1) faces += v;
+                    else edges += v;
+                }
+            });
+            const float own = s_a[wrapmod(z, 3)][0][e0];
+            st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), w0 * own + w1 * faces + w2 * edges);
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- collide
 
 // One cell of the collide pass for a 16-bit form (see lattice_fast.hpp):
@@ -302,17 +374,15 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
         });
         fdst.template st<0>(me, ((fast::fluid_rest(fl, k) + k.w0 * drho) + rf) * off_scale(0));
 
+        const float phin = k.sharp_c * phic * (1.0f - phic);
         if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
-            // Moment form of g (tau_g = 1): what collisionPhase needs.
-            st32(c.gm_u[0], me, fl.ux);
-            st32(c.gm_u[1], me, fl.uy);
-            st32(c.gm_u[2], me, fl.uz);
-            st32(c.gm_n[0], me, ncx);
-            st32(c.gm_n[1], me, ncy);
-            st32(c.gm_n[2], me, ncz);
+            // Moment form of g (tau_g = 1): B = 3 phi u + phi_n n (phase_tile_ab).
+            const float p3 = 3.0f * phic;
+            st32(c.gm_u[0], me, p3 * fl.ux + phin * ncx);
+            st32(c.gm_u[1], me, p3 * fl.uy + phin * ncy);
+            st32(c.gm_u[2], me, p3 * fl.uz + phin * ncz);
             return;
         }
-        const float phin = k.sharp_c * phic * (1.0f - phic);
         // g offset of link l after the phase collision, from its odd part.
         auto g_off = [&](auto lc, float odd) {
             constexpr int l = decltype(lc)::value;
@@ -578,7 +648,8 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
     const CaseView c = cases[w.c];
     const Geo& G = c.geo;
     if (c.gm == GM_MOMENTS) {   // uniform per workgroup
-        phase_tile_gm<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
+        if constexpr (S::kAnchored) phase_tile_ab<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
+        else phase_tile_gm<TX, TY>(c, w.x0, w.y0, w.z0, min(w.z0 + c.zc, G.nz));
         return;
     }
     // 16-bit phase populations carry the mass bookkeeping (a barrier inside).

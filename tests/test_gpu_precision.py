@@ -109,6 +109,32 @@ def test_16bit_storage_against_fp64(gpu_lib, storage, case):
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_16bit_moment_form_g_readout(gpu_lib, storage):
+    """At tau_g = 1 the 16-bit forms keep g as (phi, B): its readout is
+    initDist's w_l phi0 at init and tracks the FP32 moment form after steps;
+    writing g switches the case to stored populations."""
+    mcb = gpu_lib
+    st = mcb.STORE_FP16 if storage == "fp16" else mcb.STORE_BF16
+    shape = (24, 20, 16)
+    bub = [(12.2, 9.7, 8.1, 5.0)]
+    a = mcb.Lattice(mcb.make_params(*shape, tau_f=0.6, tau_g=1.0))
+    b = mcb.Lattice(mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, storage=st))
+    a.init_bubbles(bub, 3.0)
+    b.init_bubbles(bub, 3.0)
+    assert np.array_equal(a.read("g"), b.read("g")), "initial g = w_l phi0 in both"
+    a.step(4)
+    b.step(4)
+    ga, gb = a.read("g"), b.read("g")
+    assert np.max(np.abs(ga - gb)) <= (2e-5 if storage == "fp16" else 2e-4)
+    b.write("g", gb)
+    b.step(2)
+    a.step(2)
+    assert rel_l2(b.observables()["phi"], a.observables()["phi"]) < 1e-4
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
 def test_16bit_populations_decode_to_the_fp32_state(gpu_lib, storage):
     """Right after init the decoded populations are the reference's initial
     populations (w_l and w_l phi) to 16-bit offset precision, and writing

@@ -81,10 +81,11 @@ def test_case_bytes(mcb):
     assert mcb.case_bytes(p) == 4 * 19 * n * 4 + n * 4
     p.storage = mcb.STORE_FP16
     assert mcb.case_bytes(p) == 4 * 19 * n * 2 + 2 * n * 4
-    # tau_g = 1: f populations and g in moment form (two phi, u, n)
+    # tau_g = 1: f populations and g in moment form: two phi, and u, n (FP32)
+    # or B = 3 phi u + phi_n n (16 bits)
     p = mcb.make_params(64, 64, 64, tau_g=1.0)
     assert mcb.case_bytes(p) == 2 * 19 * n * 4 + 8 * n * 4
     p.storage = mcb.STORE_FP16
-    assert mcb.case_bytes(p) == 2 * 19 * n * 2 + 8 * n * 4
+    assert mcb.case_bytes(p) == 2 * 19 * n * 2 + 5 * n * 4
     p.tau_f = 0.4
     assert mcb.case_bytes(p) == 0   # invalid parameters
