@@ -251,6 +251,11 @@ typedef struct mcb_serve_stats {
     int completed, rejected;
     int peak_cases, rounds; /* most resident cases; scheduling rounds */
     double peak_bytes;      /* most device bytes held by resident cases */
+    /* Host time of the loop by activity, seconds: admitting cases (allocation,
+     * initial state), enqueueing steps (including batch rebuilds), waiting
+     * for the device, reading out and releasing finished cases, idle (no
+     * case resident, waiting for arrivals). */
+    double admit_s, enqueue_s, wait_s, retire_s, idle_s;
 } mcb_serve_stats;
 
 /* Replay a request trace in real time on one device. A request is admitted

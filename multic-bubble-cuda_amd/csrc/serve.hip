@@ -159,6 +159,7 @@ mcb_status serve_batched(const mcb_serve_config& cfg, const mcb_request* reqs, i
         const double now = seconds_since(t0);
         while (next < n && reqs[next].arrival_s <= now) waiting.push_back(next++);
         // Admission, in arrival order.
+        const Clock::time_point ta = Clock::now();
         while (!waiting.empty()) {
             const int i = waiting.front();
             const mcb_params p = request_params(reqs[i], cfg.storage, cfg.device);
@@ -199,16 +200,24 @@ mcb_status serve_batched(const mcb_serve_config& cfg, const mcb_request* reqs, i
             st->peak_cases = std::max(st->peak_cases, (int)active.size());
             st->peak_bytes = std::max(st->peak_bytes, (double)in_use);
         }
+        st->admit_s += seconds_since(ta);
         if (active.empty()) {
+            const Clock::time_point ti = Clock::now();
             if (next < n) wait_until(t0, reqs[next].arrival_s);
+            st->idle_s += seconds_since(ti);
             continue;
         }
         long long k = tick;
         for (const Resident& a : active) k = std::min(k, a.left);
+        const Clock::time_point te = Clock::now();
         mcb_status ss = batch.step((int)k);
+        const Clock::time_point tw = Clock::now();
+        st->enqueue_s += std::chrono::duration<double>(tw - te).count();
         if (ss == MCB_OK) ss = batch.synchronize();
+        st->wait_s += seconds_since(tw);
         if (ss != MCB_OK) return ss;
         st->rounds += 1;
+        const Clock::time_point tr = Clock::now();
         // Read out and release the cases that are done.
         for (size_t j = 0; j < active.size();) {
             Resident& a = active[j];
@@ -235,6 +244,7 @@ mcb_status serve_batched(const mcb_serve_config& cfg, const mcb_request* reqs, i
             ++settled;
             active.erase(active.begin() + (long)j);
         }
+        st->retire_s += seconds_since(tr);
     }
     finish_stats(reqs, n, res, st);
     return MCB_OK;

@@ -87,6 +87,8 @@ class ServeStats(ctypes.Structure):
         ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double),
         ("max_s", ctypes.c_double), ("completed", ctypes.c_int), ("rejected", ctypes.c_int),
         ("peak_cases", ctypes.c_int), ("rounds", ctypes.c_int), ("peak_bytes", ctypes.c_double),
+        ("admit_s", ctypes.c_double), ("enqueue_s", ctypes.c_double), ("wait_s", ctypes.c_double),
+        ("retire_s", ctypes.c_double), ("idle_s", ctypes.c_double),
     ]
 
     def as_dict(self):
