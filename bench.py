@@ -162,6 +162,8 @@ def main():
                     help="step-count scale of the serving trace (1 = the full 500..5000 steps)")
     ap.add_argument("--serve-rate", type=float, default=20.0,
                     help="moderate serving arrival rate, requests per second (latency section)")
+    ap.add_argument("--form16", choices=("fp16", "bf16"), default=None,
+                    help="only time the 256^3 case with this 16-bit storage form (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (byte-counter calibration)")
     args = ap.parse_args()
@@ -175,6 +177,13 @@ def main():
         ms = mcb.bench_copy(args.copy_probe, 3)
         print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms,
                           "gbs": 2 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if args.form16:
+        # Only the 16-bit storage form of the 256^3 case (for profiling its kernels).
+        case = mcb.case_preset("cfg2", args.seed)
+        st = mcb.STORE_FP16 if args.form16 == "fp16" else mcb.STORE_BF16
+        print(json.dumps({args.form16: storage_form(mcb, case, st, args.steps, args.warmup)}), flush=True)
+        return
         return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
