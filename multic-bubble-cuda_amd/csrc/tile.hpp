@@ -185,24 +185,43 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
         }
         hoff[q] = off;
     }
-    auto stage = [&](int z) {
+    // Staging is software-pipelined: fetch() loads a plane into registers,
+    // put() writes it to its LDS slot one iteration later, so the global
+    // loads of plane z+2 are in flight while plane z is summed.
+    float reg[KH][7];
+    auto fetch = [&](int z) {
         const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
-        float (*dst)[HN] = s_m[wrapmod(z, 3)];
 #pragma unroll
         for (int q = 0; q < KH; ++q) {
             const int e = tid + q * NT;
             if (q < KH - 1 || e < HN) {
                 const unsigned o = base + hoff[q];
-                const float phi = ld32(c.phi_prev, o);
+                reg[q][0] = ld32(c.phi_prev, o);
+                reg[q][1] = ld32(c.gm_u[0], o);
+                reg[q][2] = ld32(c.gm_u[1], o);
+                reg[q][3] = ld32(c.gm_u[2], o);
+                reg[q][4] = ld32(c.gm_n[0], o);
+                reg[q][5] = ld32(c.gm_n[1], o);
+                reg[q][6] = ld32(c.gm_n[2], o);
+            }
+        }
+    };
+    auto put = [&](int z) {
+        float (*dst)[HN] = s_m[wrapmod(z, 3)];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN) {
+                const float phi = reg[q][0];
                 dst[W1PHI][e] = w1 * phi;
                 dst[W2PHI][e] = w2 * phi;
                 dst[PHIN][e] = sc * phi * (1.0f - phi);
-                dst[UX][e] = ld32(c.gm_u[0], o);
-                dst[UY][e] = ld32(c.gm_u[1], o);
-                dst[UZ][e] = ld32(c.gm_u[2], o);
-                dst[NX][e] = ld32(c.gm_n[0], o);
-                dst[NY][e] = ld32(c.gm_n[1], o);
-                dst[NZ][e] = ld32(c.gm_n[2], o);
+                dst[UX][e] = reg[q][1];
+                dst[UY][e] = reg[q][2];
+                dst[UZ][e] = reg[q][3];
+                dst[NX][e] = reg[q][4];
+                dst[NY][e] = reg[q][5];
+                dst[NZ][e] = reg[q][6];
             }
         }
     };
@@ -210,13 +229,17 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
     const int e0 = (ty + 1) * HX + (tx + 1);
-    stage(zb - 1);
-    stage(zb);
+    fetch(zb - 1);
+    put(zb - 1);
+    fetch(zb);
+    put(zb);
+    fetch(zb + 1);
     for (int z = zb; z < ze; ++z) {
         // The slot of plane z+1 held plane z-2, last read in iteration z-1
         // before its closing barrier.
-        stage(z + 1);
+        put(z + 1);
         __syncthreads();
+        if (z + 1 < ze) fetch(z + 2);
         if (valid) {
             const unsigned me = (unsigned)(z * G.nxy + y * G.nx + x);
             float sum = 0.0f;
@@ -275,29 +298,43 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
         }
         hoff[q] = off;
     }
-    auto stage = [&](int z) {
+    // Software-pipelined staging, as in phase_tile_gm.
+    float reg[KH][4];
+    auto fetch = [&](int z) {
         const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
-        float (*dst)[HN] = s_a[wrapmod(z, 3)];
 #pragma unroll
         for (int q = 0; q < KH; ++q) {
             const int e = tid + q * NT;
             if (q < KH - 1 || e < HN) {
                 const unsigned o = base + hoff[q];
-                dst[0][e] = ld32(c.phi_prev, o);
-                dst[1][e] = ld32(c.gm_u[0], o);
-                dst[2][e] = ld32(c.gm_u[1], o);
-                dst[3][e] = ld32(c.gm_u[2], o);
+                reg[q][0] = ld32(c.phi_prev, o);
+                reg[q][1] = ld32(c.gm_u[0], o);
+                reg[q][2] = ld32(c.gm_u[1], o);
+                reg[q][3] = ld32(c.gm_u[2], o);
             }
+        }
+    };
+    auto put = [&](int z) {
+        float (*dst)[HN] = s_a[wrapmod(z, 3)];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN)
+                for (int k = 0; k < 4; ++k) dst[k][e] = reg[q][k];
         }
     };
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
     const int e0 = (ty + 1) * HX + (tx + 1);
-    stage(zb - 1);
-    stage(zb);
+    fetch(zb - 1);
+    put(zb - 1);
+    fetch(zb);
+    put(zb);
+    fetch(zb + 1);
     for (int z = zb; z < ze; ++z) {
-        stage(z + 1);
+        put(z + 1);
         __syncthreads();
+        if (z + 1 < ze) fetch(z + 2);
         if (valid) {
             float faces = 0.0f, edges = 0.0f;
             for_links([&](auto lc) {
@@ -456,6 +493,26 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
             if (q < KP - 1 || e < PN) dst[e] = ld32(src, poff[q]);
         }
     };
+    // Software-pipelined phi staging inside the z loop: fetch_phi() loads a
+    // plane into registers while the previous plane's cells are computed,
+    // put_phi() writes it to its LDS slot at the top of the next iteration.
+    float preg[KP];
+    auto fetch_phi = [&](int z) {
+        const float* __restrict__ src = c.phi + (size_t)wrapmod(z, G.nz) * G.nxy;
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) preg[q] = ld32(src, poff[q]);
+        }
+    };
+    auto put_phi = [&](int z) {
+        float* dst = s_phi[z & 3];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = preg[q];
+        }
+    };
     auto nslot = [](int z) { return (z + 3) % 3; };  // z >= -3
     auto comp_n = [&](int z) {
         const float* pm = s_phi[(z - 1) & 3];
@@ -498,6 +555,7 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     load_phi(zb - 1);
     load_phi(zb);
     load_phi(zb + 1);
+    fetch_phi(zb + 2);
     __syncthreads();
     comp_n(zb - 1);
     comp_n(zb);
@@ -506,11 +564,12 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     for (int z = zb; z < ze; ++z) {
         // Slot (z+2)&3 held plane z-2, last read by comp_n(z-1) before the
         // previous barrier.
-        load_phi(z + 2);
+        put_phi(z + 2);
         __syncthreads();
         // n slot of z+1 held plane z-2 (read last in iteration z-1).
         comp_n(z + 1);
         __syncthreads();
+        if (z + 1 < ze) fetch_phi(z + 3);
         if (!valid) continue;
 
         const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
