@@ -40,6 +40,29 @@ def test_driver_reference_preset_matches_oracle(gpu_lib, tmp_path):
         assert np.array_equal(got, want), f"step {step}"
 
 
+def test_driver_periodic_moment_form_matches_oracle(gpu_lib, tmp_path):
+    """cfg0 with tau_g = 1 runs the fused path with g in moment form; its phi
+    files equal the oracle's d_phi bit for bit."""
+    mcb = gpu_lib
+    root = str(tmp_path) + "/"
+    r = subprocess.run([mcb.DRIVER_PATH, "D3Q19", "gm", "--preset", "cfg0", "--tau-g", "1", "--steps", "40",
+                        "--save-every", "20", "--root", root], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sim = root + "bin/D3Q19/gm/"
+    assert postfmt.macr_steps(sim) == [0, 20, 40]
+    c = mcb.case_preset("cfg0")
+    c.params.tau_g = 1.0
+    o = oracle_for(c.params)
+    o.init(f32_bubbles(mcb.case_bubbles(c)), c.width)
+    done = 0
+    for step in (0, 20, 40):
+        o.step(step + 1 - done)
+        done = step + 1
+        got = postfmt.read_macr(postfmt.macr_file(sim, "gm", "phi", step), sim)
+        want = np.swapaxes(o.field("phi"), 0, 2)
+        assert np.array_equal(got, want), f"step {step}"
+
+
 def test_driver_periodic_preset_with_extra_fields(gpu_lib, tmp_path):
     mcb = gpu_lib
     root = str(tmp_path) + "/"
