@@ -97,11 +97,12 @@ Tiling choose_tiling(const mcb_params& p) {
     }
     Tiling t = p.nx >= 64 ? Tiling{64, 8, 16} : (p.nx >= 32 ? Tiling{32, 8, 16} : Tiling{16, 16, 16});
     // 16-bit forms: half the bytes per cell, so longer z runs pay off
-    // (measured at 256^3: 64x4x32 fastest); a 32-wide tile where it pads x
-    // less (e.g. nx = 80, 144, 160), since padded lanes do no work.
+    // (measured at 256^3: 64x8x32 fastest, 64x4x32 4% and 32x8x32 17%
+    // slower); a 32-wide tile where it pads x less (e.g. nx = 80, 144, 160),
+    // since padded lanes do no work.
     if (p.storage != MCB_STORE_FP32 && p.nx >= 48) {
         const int pad64 = (p.nx + 63) / 64 * 64, pad32 = (p.nx + 31) / 32 * 32;
-        t = pad32 < pad64 ? Tiling{32, 8, 32} : Tiling{64, 4, 32};
+        t = pad32 < pad64 ? Tiling{32, 8, 32} : Tiling{64, 8, 32};
     }
     t.zc = std::min(t.zc, p.nz);
     return t;

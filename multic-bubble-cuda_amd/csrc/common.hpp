@@ -48,6 +48,14 @@ __device__ __forceinline__ void st32(float* __restrict__ base, unsigned elem, fl
     const g_fp p = (g_fp)base;
     *(g_fp)((char __attribute__((address_space(1)))*)p + (elem << 2)) = v;
 }
+// The same with a byte offset that the caller keeps precomputed (no shift
+// per access). Callers fold wave-uniform parts (the plane) into base.
+__device__ __forceinline__ float ld32b(const float* __restrict__ base, unsigned byte_off) {
+    return *(gconst_fp)((const char __attribute__((address_space(1)))*)(gconst_fp)base + byte_off);
+}
+__device__ __forceinline__ void st32b(float* __restrict__ base, unsigned byte_off, float v) {
+    *(g_fp)((char __attribute__((address_space(1)))*)(g_fp)base + byte_off) = v;
+}
 
 // The 19 population planes of one buffer behind a single buffer resource:
 // base address in four scalar registers, plane l selected by the scalar
@@ -81,6 +89,33 @@ struct Planes {
     template <int L> __device__ __forceinline__ void st(unsigned elem, float v) const {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, elem << 2, L * plane_bytes,
                                               MCB_STORE_AUX);
+    }
+    // Split addressing: vb = per-lane byte offset within a plane (loop
+    // invariant), sb = wave-uniform byte offset of the z plane, which goes
+    // into the scalar offset next to the population plane.
+    template <int L> __device__ __forceinline__ float ldb(unsigned vb, unsigned sb) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vb, L * plane_bytes + sb, 0));
+    }
+    template <int L> __device__ __forceinline__ void stb(unsigned vb, unsigned sb, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vb, L * plane_bytes + sb,
+                                              MCB_STORE_AUX);
+    }
+};
+
+// One FP32 field behind a buffer resource, addressed like Planes::ldb / stb:
+// a per-lane byte offset that the caller keeps (loop-invariant) and a
+// wave-uniform byte offset (the z plane), so no 64-bit per-lane address is
+// formed per access. bytes: the field's extent (accesses beyond it read 0 and
+// drop stores).
+struct Field {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Field(const float* base, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, kBufferConfigWord)) {}
+    __device__ __forceinline__ float ld(unsigned vb, unsigned sb) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vb, sb, 0));
+    }
+    __device__ __forceinline__ void st(unsigned vb, unsigned sb, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vb, sb, 0);
     }
 };
 

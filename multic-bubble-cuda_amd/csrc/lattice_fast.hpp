@@ -5,44 +5,46 @@
 // for bit, so this is only used where populations are stored in 16 bits.
 //
 // D3Q19 links come in opposite pairs (2k-1, 2k), k = 1..9, with c_{2k} =
-// -c_{2k-1}. With a = c.u / cs^2, b = 1.5 (c.F) / rho, beta = 1.5 (u.F) / rho,
-// e = a^2 / 2 - 1.5 |u|^2 and o = 1 + e, the reference's quantities of the
-// pair are
-//   feq_+- - HeF_+- = w rho (o +- a)(1 + beta -+ b)      (momentiCalc, lbm.cu:200-212)
-//   feq_+- + HeF_+- = w rho (o +- a)(1 - beta +- b)      (collisionFluid, lbm.cu:273-286)
-// so each pair needs one even part E = w rho (o (1 -+ beta) +- a b) and one
-// odd part O, and the regularised non-equilibrium term (Q_l : Pi, even in l)
-// once. Pi only needs the even parts: fneq_+ + fneq_- = f_+ + f_- - 2 E.
-// The phase collision (lbm.cu:319-326) splits the same way:
+// -c_{2k-1}. With a = c.u / cs^2, b = 1.5 (c.F) / rho, beta = 1.5 (u.F) / rho
+// and o = 1 + a^2/2 - 1.5 |u|^2 of a link, the reference's quantities are
+//   feq - HeF = w rho (o + a)(1 + beta - b)      (momentiCalc, lbm.cu:200-212)
+//   feq + HeF = w rho (o + a)(1 - beta + b)      (collisionFluid, lbm.cu:273-286)
+// and a link's opposite flips the signs of a and b. The fluid collision is
+// evaluated as a polynomial in c_l (fluid_poly); the phase collision
+// (lbm.cu:319-326) splits into an even part w phi and an odd part:
 // g*_+- = w phi +- w (phi a + phi_norm c.n).
 //
-// Post-collision values are returned as offsets from w_l * anchor (rho for f,
-// phi for g): the small part is formed directly, without the cancellation of
-// subtracting w_l * anchor from a full population.
+// Post-collision values are returned as offsets (f*_l / w_l - 1 for f, the
+// odd part for g): the small part is formed directly, without the
+// cancellation of subtracting w_l * anchor from a full population.
 #pragma once
 #include "lattice.hpp"
 
 namespace mcb {
 namespace fast {
 
-// Pair k = 1..9: its "+" link is 2k-1.
-template <int K> constexpr int PL = 2 * K - 1;
-
 struct Consts {
     float w0, w1, w2;
     float coef1, coef2;            // 3 w1, 3 w2 (gradient / curvature weights)
-    float r0, r1, r2;              // (1 - omega) w_c / (2 cs^4)
     float sharp_c, sigma, omega_g, one_m_omega_g;
-    __device__ __forceinline__ explicit Consts(const Phys& p) {
+    // Coefficients of the polynomial form (fluid_poly).
+    float m_rho, m_uf, m_x1, m_t, m_t2, tr_c;
+    float wk1, wk2;                // w_c * K_c: 16-bit offset scale of f*_l / w_l - 1
+    __device__ __forceinline__ explicit Consts(const Phys& p, float k1 = 1.0f, float k2 = 1.0f) {
+        wk1 = p.w[1] * k1;
+        wk2 = p.w[2] * k2;
+        const float w = p.omega;
+        m_rho = 4.5f * w;                 // 9/2 omega
+        m_uf = 6.75f * (2.0f - w);        // 27/4 (2 - omega)
+        m_x1 = 4.5f * (2.0f - w);
+        m_t = 4.5f * (1.0f - w);
+        m_t2 = 9.0f * (1.0f - w);
+        tr_c = 1.5f * (1.0f - w);
         w0 = p.w[0];
         w1 = p.w[1];
         w2 = p.w[2];
         coef1 = 3.0f * w1;
         coef2 = 3.0f * w2;
-        const float om = 1.0f - p.omega;
-        r0 = om * (w0 / TWO_CS4);
-        r1 = om * (w1 / TWO_CS4);
-        r2 = om * (w2 / TWO_CS4);
         sharp_c = p.sharp_c;
         sigma = p.sigma;
         omega_g = p.omega_g;
@@ -94,110 +96,114 @@ __device__ __forceinline__ void curvature_force(NBN&& nbn, const Consts& k, floa
     fz = m * nz;
 }
 
-// Moments of the fluid populations and the per-cell terms of the collision.
-struct Fluid {
-    float rho, ux, uy, uz;
-    float uu, beta, hb, wr0, wr1, wr2;  // 1.5|u|^2, beta, 1.5/rho, w_c rho
-    float pxx, pyy, pzz, pxy, pxz, pyz;
-    float trc;                           // cs^2 (pxx + pyy + pzz)
-};
-
-template <int L> __device__ __forceinline__ float wr_of(const Fluid& f) {
-    return WC[L] == 0 ? f.wr0 : (WC[L] == 1 ? f.wr1 : f.wr2);
-}
-
-// a = c.u / cs^2, b = 1.5 c.F / rho and o = 1 + a^2/2 - 1.5|u|^2 of link L.
-template <int L>
-__device__ __forceinline__ void pair_terms(const Fluid& f, float fx, float fy, float fz, float& a, float& b,
-                                           float& o) {
-    a = 3.0f * cdot<L>(f.ux, f.uy, f.uz);
-    b = f.hb * cdot<L>(fx, fy, fz);
-    o = 1.0f + (0.5f * a * a - f.uu);
-}
-
-__device__ __forceinline__ Fluid fluid(const float (&fv)[Q], const Consts& k, float fx, float fy, float fz) {
-    Fluid f;
-    float s[10], dd[10];
-    s[0] = fv[0];
-    dd[0] = 0.0f;
-    float rho = fv[0];
+// momentiCalc + collisionFluid (lbm.cu:133-288) for one cell as a polynomial
+// in c_l. Input: the offsets t_l = f_l - w_l and the force F. With
+// J = sum_l c_l f_l + F/2 = rho u, uF = u.F, rho_m = rho - 1.5 uF and
+// rho_p = rho + 1.5 uF, the reference's post-collision population is
+//   f*_l = w_l (rho (o + a)(1 - beta + b) + (9/2)(1 - omega)(c c : Pi - tr Pi / 3))
+// (a, b, o, beta as above), which expands to
+//   f*_l / w_l = P0 + c c : M + c.V + (27/4) (c.u)^2 (c.F)
+// with
+//   P0   = rho_m (1 - 1.5|u|^2) - 1.5 (1 - omega) tr Pi
+//   M_ab = (9/2) [rho_m u_a u_b + X_ab + (1 - omega) Pi_ab],  X_ab = (u_a F_b + u_b F_a)/2
+//   V    = 3 rho_m u + 1.5 (1 - 1.5|u|^2) F.
+// Pi (momentiCalc's sum of c c (f - feq + HeF)) has a closed form through
+// D3Q19's moment identities (sum w c_a c_b = delta/3, fourth moment
+// (delta delta + delta delta + delta delta)/9): with T_ab = sum_l c_a c_b t_l,
+//   Pi_ab = T_ab - rho_p u_a u_b + X_ab - delta_ab (rho - 1)/3,
+// so M_ab = (9/2)[(omega rho - 1.5 (2 - omega) uF) u_a u_b + (2 - omega) X_ab
+// + (1 - omega) T_ab] - delta_ab 1.5 (1 - omega)(rho - 1). On D3Q19 links
+// (components 0, +-1) the cubic term is linear in the signs: a face link
+// +-e_p gives +-u_p^2 F_p, an edge link s_p e_p + s_q e_q gives
+// s_p (F_p (u_p^2 + u_q^2) + 2 u_p u_q F_q) + s_q (F_q (u_p^2 + u_q^2) + 2 u_p u_q F_p).
+// So every link is one even part (shared by the pair) plus or minus one odd
+// part, both sums of per-cell coefficients: a few adds per link instead of
+// the per-link products of the reference's form. Same model, other rounding.
+//
+// emit(LinkC<l>, x) receives f*_l / w_l - 1 for l = 1..18; the function
+// returns it for l = 0 and sets rho and u.
+template <class Emit>
+__device__ __forceinline__ float fluid_poly(const float (&t)[Q], const Consts& k, float fx, float fy, float fz,
+                                           float& rho, float& ux, float& uy, float& uz, Emit&& emit) {
+    // Pairs k = 1..9 (links 2k-1, 2k): x, y, z, (1,1,0), (1,0,1), (0,1,1),
+    // (1,-1,0), (1,0,-1), (0,1,-1).
+    float s[10], d[10];
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         if constexpr (l % 2 == 1) {
-            s[(l + 1) / 2] = fv[l] + fv[l + 1];
-            dd[(l + 1) / 2] = fv[l] - fv[l + 1];
+            static_assert(CX[l] == -CX[l + 1] && CY[l] == -CY[l + 1] && CZ[l] == -CZ[l + 1], "opposite pairs");
+            s[(l + 1) / 2] = t[l] + t[l + 1];
+            d[(l + 1) / 2] = t[l] - t[l + 1];
         }
     });
-    rho = (rho + (s[1] + s[2] + s[3])) + ((s[4] + s[5] + s[6]) + (s[7] + s[8] + s[9]));
-    // pairs: 1 x, 2 y, 3 z, 4 (1,1,0), 5 (1,0,1), 6 (0,1,1), 7 (1,-1,0), 8 (1,0,-1), 9 (0,1,-1)
-    const float jx = dd[1] + (dd[4] + dd[5]) + (dd[7] + dd[8]);
-    const float jy = dd[2] + (dd[4] + dd[6]) + (dd[9] - dd[7]);
-    const float jz = dd[3] + (dd[5] + dd[6]) - (dd[8] + dd[9]);
+    static_assert(CX[7] == 1 && CY[7] == 1 && CX[9] == 1 && CZ[9] == 1 && CY[11] == 1 && CZ[11] == 1 &&
+                      CX[13] == 1 && CY[13] == -1 && CX[15] == 1 && CZ[15] == -1 && CY[17] == 1 && CZ[17] == -1,
+                  "pair table");
+    const float drho = (t[0] + (s[1] + s[2] + s[3])) + ((s[4] + s[5] + s[6]) + (s[7] + s[8] + s[9]));
+    rho = 1.0f + drho;
+    const float jx = (d[1] + (d[4] + d[5])) + (d[7] + d[8]);
+    const float jy = (d[2] + (d[4] + d[6])) + (d[9] - d[7]);
+    const float jz = (d[3] + (d[5] + d[6])) - (d[8] + d[9]);
+    const float txx = (s[1] + (s[4] + s[5])) + (s[7] + s[8]);
+    const float tyy = (s[2] + (s[4] + s[6])) + (s[7] + s[9]);
+    const float tzz = (s[3] + (s[5] + s[6])) + (s[8] + s[9]);
+    const float txy = s[4] - s[7], txz = s[5] - s[8], tyz = s[6] - s[9];
+
     const float ir = __builtin_amdgcn_rcpf(rho);
-    f.rho = rho;
-    f.ux = (jx + 0.5f * fx) * ir;
-    f.uy = (jy + 0.5f * fy) * ir;
-    f.uz = (jz + 0.5f * fz) * ir;
-    f.uu = 1.5f * (f.ux * f.ux + f.uy * f.uy + f.uz * f.uz);
-    f.hb = 1.5f * ir;
-    f.beta = f.hb * (f.ux * fx + f.uy * fy + f.uz * fz);
-    f.wr0 = k.w0 * rho;
-    f.wr1 = k.w1 * rho;
-    f.wr2 = k.w2 * rho;
-    // fneq_+ + fneq_- of each pair: s - 2 w rho (o (1 + beta) - a b).
-    float S[10];
-    const float opb = 1.0f + f.beta;
-    for_links([&](auto lc) {
+    ux = (jx + 0.5f * fx) * ir;
+    uy = (jy + 0.5f * fy) * ir;
+    uz = (jz + 0.5f * fz) * ir;
+    const float uf = ux * fx + uy * fy + uz * fz;
+    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    const float u2 = ux2 + uy2 + uz2;
+    const float pxy = ux * uy, pxz = ux * uz, pyz = uy * uz;
+    const float rho_m = rho - 1.5f * uf, rho_p = rho + 1.5f * uf;
+
+    // M (diagonal) and 2 M (off-diagonal).
+    const float c1 = k.m_rho * rho - k.m_uf * uf;
+    const float c2 = c1 + c1;
+    const float dd = -k.tr_c * drho;
+    const float mxx = (c1 * ux2 + (k.m_x1 * (ux * fx) + k.m_t * txx)) + dd;
+    const float myy = (c1 * uy2 + (k.m_x1 * (uy * fy) + k.m_t * tyy)) + dd;
+    const float mzz = (c1 * uz2 + (k.m_x1 * (uz * fz) + k.m_t * tzz)) + dd;
+    const float m2xy = c2 * pxy + (k.m_x1 * (ux * fy + uy * fx) + k.m_t2 * txy);
+    const float m2xz = c2 * pxz + (k.m_x1 * (ux * fz + uz * fx) + k.m_t2 * txz);
+    const float m2yz = c2 * pyz + (k.m_x1 * (uy * fz + uz * fy) + k.m_t2 * tyz);
+    const float tr_pi = ((txx + tyy + tzz) - rho_p * u2) + (uf - drho);
+    const float p0m1 = ((drho - 1.5f * uf) - rho_m * (1.5f * u2)) - k.tr_c * tr_pi;
+
+    // Odd coefficients.
+    const float r3 = 3.0f * rho_m, hv = 1.5f - 2.25f * u2;
+    const float vx = r3 * ux + hv * fx, vy = r3 * uy + hv * fy, vz = r3 * uz + hv * fz;
+    const float afx = vx + 6.75f * (ux2 * fx), afy = vy + 6.75f * (uy2 * fy), afz = vz + 6.75f * (uz2 * fz);
+    auto plane = [&](float vp, float vq, float up2, float uq2, float p2, float fp, float fq, float& ap, float& aq) {
+        const float sq = up2 + uq2;
+        ap = vp + 6.75f * (fp * sq + p2 * fq);
+        aq = vq + 6.75f * (fq * sq + p2 * fp);
+    };
+    float axy_x, axy_y, axz_x, axz_z, ayz_y, ayz_z;
+    plane(vx, vy, ux2, uy2, pxy + pxy, fx, fy, axy_x, axy_y);
+    plane(vx, vz, ux2, uz2, pxz + pxz, fx, fz, axz_x, axz_z);
+    plane(vy, vz, uy2, uz2, pyz + pyz, fy, fz, ayz_y, ayz_z);
+
+    // Even parts.
+    const float efx = p0m1 + mxx, efy = p0m1 + myy, efz = p0m1 + mzz;
+    const float exy = efx + myy, exz = efx + mzz, eyz = efy + mzz;
+    auto pair = [&](auto lc, float even, float odd) {
         constexpr int l = decltype(lc)::value;
-        if constexpr (l % 2 == 1) {
-            float a, b, o;
-            pair_terms<l>(f, fx, fy, fz, a, b, o);
-            const float E = wr_of<l>(f) * (o * opb - a * b);
-            S[(l + 1) / 2] = s[(l + 1) / 2] - 2.0f * E;
-        }
-    });
-    f.pxx = S[1] + (S[4] + S[5]) + (S[7] + S[8]);
-    f.pyy = S[2] + (S[4] + S[6]) + (S[7] + S[9]);
-    f.pzz = S[3] + (S[5] + S[6]) + (S[8] + S[9]);
-    f.pxy = S[4] - S[7];
-    f.pxz = S[5] - S[8];
-    f.pyz = S[6] - S[9];
-    f.trc = CSSQ * (f.pxx + f.pyy + f.pzz);
-    return f;
-}
-
-// Q_l : Pi for the "+" link L of a pair (the same for its opposite link).
-template <int L> __device__ __forceinline__ float q_pi(const Fluid& f) {
-    constexpr int cx = CX[L], cy = CY[L], cz = CZ[L];
-    float s = -f.trc;
-    if constexpr (cx != 0) s = s + f.pxx;
-    if constexpr (cy != 0) s = s + f.pyy;
-    if constexpr (cz != 0) s = s + f.pzz;
-    if constexpr (cx * cy != 0) s = cx * cy > 0 ? s + 2.0f * f.pxy : s - 2.0f * f.pxy;
-    if constexpr (cx * cz != 0) s = cx * cz > 0 ? s + 2.0f * f.pxz : s - 2.0f * f.pxz;
-    if constexpr (cy * cz != 0) s = cy * cz > 0 ? s + 2.0f * f.pyz : s - 2.0f * f.pyz;
-    return s;
-}
-
-// Post-collision offsets f*_l - w_l rho of the pair whose "+" link is L.
-template <int L>
-__device__ __forceinline__ void fluid_pair(const Fluid& f, const Consts& k, float fx, float fy, float fz, float& op,
-                                           float& om) {
-    float a, b, o;
-    pair_terms<L>(f, fx, fy, fz, a, b, o);
-    const float wr = wr_of<L>(f);
-    const float omb = 1.0f - f.beta;
-    // w rho (o (1 - beta) + a b - 1) = w rho ((o - 1) - beta o + a b)
-    const float even = wr * (((o - 1.0f) - f.beta * o) + a * b) + (WC[L] == 1 ? k.r1 : k.r2) * q_pi<L>(f);
-    const float odd = wr * (a * omb + b * o);
-    op = even + odd;
-    om = even - odd;
-}
-
-// Rest link: f*_0 - w0 rho.
-__device__ __forceinline__ float fluid_rest(const Fluid& f, const Consts& k) {
-    // w0 rho ((1 - uu)(1 - beta) - 1) + r0 (-trc)
-    return f.wr0 * (-f.uu - f.beta * (1.0f - f.uu)) - k.r0 * f.trc;
+        emit(LinkC<l>{}, even + odd);
+        emit(LinkC<l + 1>{}, even - odd);
+    };
+    pair(LinkC<1>{}, efx, afx);
+    pair(LinkC<3>{}, efy, afy);
+    pair(LinkC<5>{}, efz, afz);
+    pair(LinkC<7>{}, exy + m2xy, axy_x + axy_y);
+    pair(LinkC<9>{}, exz + m2xz, axz_x + axz_z);
+    pair(LinkC<11>{}, eyz + m2yz, ayz_y + ayz_z);
+    pair(LinkC<13>{}, exy - m2xy, axy_x - axy_y);
+    pair(LinkC<15>{}, exz - m2xz, axz_x - axz_z);
+    pair(LinkC<17>{}, eyz - m2yz, ayz_y - ayz_z);
+    return p0m1;
 }
 
 // Phase collision, odd part w (phi a + phi_norm c.n) of the pair whose "+"

@@ -93,8 +93,11 @@ struct StoreBF16 {
 // in common.hpp); 16-bit forms move raw offsets.
 template <class S> struct PopPlanes;
 template <> struct PopPlanes<StoreF32> : Planes {
+    using Raw = float;   // a population as loaded (see raw())
     __device__ __forceinline__ PopPlanes(const void* base, unsigned plane_elems)
         : Planes(static_cast<const float*>(base), plane_elems) {}
+    template <int L> __device__ __forceinline__ Raw raw(unsigned vb, unsigned sb) const { return ldb<L>(vb, sb); }
+    static __device__ __forceinline__ float dec(Raw v) { return v; }
 };
 template <class S> struct PopPlanes {
     __amdgpu_buffer_rsrc_t r;
@@ -112,6 +115,38 @@ template <class S> struct PopPlanes {
         __builtin_amdgcn_raw_buffer_store_b16(b, r, elem << 1, L * plane_bytes, MCB_STORE16_AUX);
         return S::dec(b);
     }
+    // Split addressing, as Planes::ldb / stb.
+    template <int L> __device__ __forceinline__ float ldb(unsigned vb, unsigned sb) const {
+        return S::dec(__builtin_amdgcn_raw_buffer_load_b16(r, vb, L * plane_bytes + sb, 0));
+    }
+    // The stored 16 bits, decoded later with dec(): lets a caller issue the
+    // loads of the next plane early and keep them in flight as raw bits.
+    using Raw = unsigned short;
+    template <int L> __device__ __forceinline__ Raw raw(unsigned vb, unsigned sb) const {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, vb, L * plane_bytes + sb, 0);
+    }
+    static __device__ __forceinline__ float dec(Raw b) { return S::dec(b); }
+    template <int L> __device__ __forceinline__ float stb(unsigned vb, unsigned sb, float v) const {
+        const unsigned short b = S::enc(v);
+        __builtin_amdgcn_raw_buffer_store_b16(b, r, vb, L * plane_bytes + sb, MCB_STORE16_AUX);
+        return S::dec(b);
+    }
+};
+
+// Byte offsets within a plane of the 9 columns (x - c_x, y - c_y) that a
+// cell's pull-form gathers read, for elements of EB bytes. They do not depend
+// on z, so a tile computes them once; per plane only the wave-uniform plane
+// offset changes, and it goes into the scalar offset of the access.
+template <unsigned EB> struct Columns {
+    unsigned b[9];
+    __device__ __forceinline__ Columns(const unsigned (&xs)[3], const unsigned (&ys)[3]) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) b[j * 3 + i] = (ys[j] + xs[i]) * EB;
+    }
+    template <int L> __device__ __forceinline__ unsigned src() const { return b[(1 - CY[L]) * 3 + (1 - CX[L])]; }
+    __device__ __forceinline__ unsigned me() const { return b[4]; }
 };
 
 // ------------------------------------------------------------------ phase
@@ -359,65 +394,61 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
 
 // ---------------------------------------------------------------- collide
 
-// One cell of the collide pass for a 16-bit form (see lattice_fast.hpp):
-// surface-tension force, moments, and the post-collision offsets of f,
+// One cell of the collide pass for a 16-bit form (see lattice_fast.hpp), from
+// its gathered f offsets fr (raw 16-bit): surface-tension force, moments, and
+// the post-collision offsets of f,
 // written against w_l, and of g, written against w_l phi of this cell (or, in
 // moment form, this cell's u and n). Links 1..18 go first; their rounding
 // residuals (in population units) then go into the rest link.
 template <int MODE, class S, class NBN>
 __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k, const PopPlanes<S>& fsrc,
                                        const PopPlanes<S>& gsrc, const PopPlanes<S>& fdst,
-                                       const PopPlanes<S>& gdst, unsigned me, const unsigned (&xs)[3],
-                                       const unsigned (&ys)[3], const unsigned (&zs)[3], NBN&& nbn, float ncx,
-                                       float ncy, float ncz, float ind, float phic) {
+                                       const PopPlanes<S>& gdst, const Columns<2>& col, unsigned me4,
+                                       const unsigned (&zs)[3], const unsigned short (&fr)[Q], NBN&& nbn,
+                                       float ncx, float ncy, float ncz, float ind, float phic) {
     float Fx, Fy, Fz;
     fast::curvature_force(nbn, k, ncx, ncy, ncz, ind, Fx, Fy, Fz);
-    float fv[Q];
+    float tv[Q];   // f_l - w_l
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        fv[l] = pop_from_offset<l>(c.phys, 1.0f, fsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]));
+        tv[l] = S::dec(fr[l]) * off_unscale(WC[l]);
     });
-    const fast::Fluid fl = fast::fluid(fv, k, Fx, Fy, Fz);
+    const unsigned z1 = zs[1];
+    const unsigned fb = (unsigned)(c.geo.n * 4);   // bytes of one FP32 field
+    // Stores the 16-bit form of t = offset * K and adds its rounding residual
+    // (population units) to r.
+    auto put = [&](auto lc, const PopPlanes<S>& dst, float t, float& r) {
+        constexpr int l = decltype(lc)::value;
+        r += (t - dst.template stb<l>(col.me(), z1 * 2u, t)) * off_unscale(WC[l]);
+    };
+    float rf = 0.0f;
+    auto emit = [&](auto lc, float x) {   // x = f*_l / w_l - 1
+        constexpr int l = decltype(lc)::value;
+        if constexpr (MODE != COLLIDE_MACROS) put(lc, fdst, x * (WC[l] == 1 ? k.wk1 : k.wk2), rf);
+    };
+    float rho, ux, uy, uz;
+    const float x0 = fast::fluid_poly(tv, k, Fx, Fy, Fz, rho, ux, uy, uz, emit);
     if constexpr (MODE == COLLIDE_MACROS) {
-        st32(c.rho, me, fl.rho);
-        st32(c.ux, me, fl.ux);
-        st32(c.uy, me, fl.uy);
-        st32(c.uz, me, fl.uz);
-        st32(c.fx, me, Fx);
-        st32(c.fy, me, Fy);
-        st32(c.fz, me, Fz);
-        st32(c.nx, me, ncx);
-        st32(c.ny, me, ncy);
-        st32(c.nz, me, ncz);
+        Field(c.rho, fb).st(me4, z1 * 4u, rho);
+        Field(c.ux, fb).st(me4, z1 * 4u, ux);
+        Field(c.uy, fb).st(me4, z1 * 4u, uy);
+        Field(c.uz, fb).st(me4, z1 * 4u, uz);
+        Field(c.fx, fb).st(me4, z1 * 4u, Fx);
+        Field(c.fy, fb).st(me4, z1 * 4u, Fy);
+        Field(c.fz, fb).st(me4, z1 * 4u, Fz);
+        Field(c.nx, fb).st(me4, z1 * 4u, ncx);
+        Field(c.ny, fb).st(me4, z1 * 4u, ncy);
+        Field(c.nz, fb).st(me4, z1 * 4u, ncz);
     } else {
-        auto put = [&](auto lc, const PopPlanes<S>& dst, float off, float& r) {
-            constexpr int l = decltype(lc)::value;
-            const float t = off * off_scale(WC[l]);
-            r += (t - dst.template st<l>(me, t)) * off_unscale(WC[l]);
-        };
-        // f*_l - w_l = (f*_l - w_l rho) + w_l (rho - 1)
-        const float drho = fl.rho - 1.0f;
-        const float dr1 = k.w1 * drho, dr2 = k.w2 * drho;
-        float rf = 0.0f;
-        for_links([&](auto lc) {
-            constexpr int l = decltype(lc)::value;
-            if constexpr (l % 2 == 1) {
-                float op, om;
-                fast::fluid_pair<l>(fl, k, Fx, Fy, Fz, op, om);
-                const float dr = WC[l] == 1 ? dr1 : dr2;
-                put(LinkC<l>{}, fdst, op + dr, rf);
-                put(LinkC<l + 1>{}, fdst, om + dr, rf);
-            }
-        });
-        fdst.template st<0>(me, ((fast::fluid_rest(fl, k) + k.w0 * drho) + rf) * off_scale(0));
+        fdst.template stb<0>(col.me(), z1 * 2u, (k.w0 * x0 + rf) * off_scale(0));
 
         const float phin = k.sharp_c * phic * (1.0f - phic);
         if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
             // Moment form of g (tau_g = 1): B = 3 phi u + phi_n n (phase_tile_ab).
             const float p3 = 3.0f * phic;
-            st32(c.gm_u[0], me, p3 * fl.ux + phin * ncx);
-            st32(c.gm_u[1], me, p3 * fl.uy + phin * ncy);
-            st32(c.gm_u[2], me, p3 * fl.uz + phin * ncz);
+            Field(c.gm_u[0], fb).st(me4, z1 * 4u, p3 * ux + phin * ncx);
+            Field(c.gm_u[1], fb).st(me4, z1 * 4u, p3 * uy + phin * ncy);
+            Field(c.gm_u[2], fb).st(me4, z1 * 4u, p3 * uz + phin * ncz);
             return;
         }
         // g offset of link l after the phase collision, from its odd part.
@@ -428,9 +459,9 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
             } else {
                 // (1 - omega_g)(g_pre - w phi) + omega_g (+-odd), with
                 // g_pre - w phi = w (anchor - phi) + t / K.
-                const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
-                const float pre = __builtin_fmaf(k.template w<l>(), ld32(c.phi_prev, src) - phic,
-                                                 gsrc.template ld<l>(src) * off_unscale(WC[l]));
+                const unsigned vb = col.template src<l>(), zb = zs[1 - CZ[l]];
+                const float pre = __builtin_fmaf(k.template w<l>(), Field(c.phi_prev, fb).ld(vb * 2u, zb * 4u) - phic,
+                                                 gsrc.template ldb<l>(vb, zb * 2u) * off_unscale(WC[l]));
                 return k.one_m_omega_g * pre + k.omega_g * odd;
             }
         };
@@ -438,12 +469,12 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
         for_links([&](auto lc) {
             constexpr int l = decltype(lc)::value;
             if constexpr (l % 2 == 1) {
-                const float odd = fast::phase_odd<l>(k, phic, phin, fl.ux, fl.uy, fl.uz, ncx, ncy, ncz);
-                put(LinkC<l>{}, gdst, g_off(LinkC<l>{}, odd), rg);
-                put(LinkC<l + 1>{}, gdst, g_off(LinkC<l + 1>{}, -odd), rg);
+                const float odd = fast::phase_odd<l>(k, phic, phin, ux, uy, uz, ncx, ncy, ncz);
+                put(LinkC<l>{}, gdst, g_off(LinkC<l>{}, odd) * off_scale(WC[l]), rg);
+                put(LinkC<l + 1>{}, gdst, g_off(LinkC<l + 1>{}, -odd) * off_scale(WC[l]), rg);
             }
         });
-        gdst.template st<0>(me, (g_off(LinkC<0>{}, 0.0f) + rg) * off_scale(0));
+        gdst.template stb<0>(col.me(), z1 * 2u, (g_off(LinkC<0>{}, 0.0f) + rg) * off_scale(0));
     }
 }
 
@@ -467,11 +498,11 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     const Geo& G = c.geo;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const Derived d(c.phys);
-    const fast::Consts k16(c.phys);
+    const fast::Consts k16(c.phys, off_scale(1), off_scale(2));
     const PopPlanes<S> fsrc(c.fsrc, (unsigned)G.plane), fdst(c.fdst, (unsigned)G.plane);
     const PopPlanes<S> gsrc(c.gsrc, (unsigned)G.plane), gdst(c.gdst, (unsigned)G.plane);
 
-    // In-plane offsets of the halo phi elements this thread stages.
+    // In-plane byte offsets of the halo phi elements this thread stages.
     unsigned poff[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) {
@@ -479,18 +510,19 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
         unsigned off = 0;
         if (e < PN) {
             const int lx = e % PX, ly = e / PX;
-            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx));
+            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx)) * 4u;
         }
         poff[q] = off;
     }
 
+    const Field phi_f(c.phi, (unsigned)(G.n * 4));
     auto load_phi = [&](int z) {
-        const float* __restrict__ src = c.phi + (size_t)wrapmod(z, G.nz) * G.nxy;
+        const unsigned zb4 = (unsigned)(wrapmod(z, G.nz) * G.nxy) * 4u;
         float* dst = s_phi[z & 3];
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
             const int e = tid + q * NT;
-            if (q < KP - 1 || e < PN) dst[e] = ld32(src, poff[q]);
+            if (q < KP - 1 || e < PN) dst[e] = phi_f.ld(poff[q], zb4);
         }
     };
     // Software-pipelined phi staging inside the z loop: fetch_phi() loads a
@@ -498,11 +530,11 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     // put_phi() writes it to its LDS slot at the top of the next iteration.
     float preg[KP];
     auto fetch_phi = [&](int z) {
-        const float* __restrict__ src = c.phi + (size_t)wrapmod(z, G.nz) * G.nxy;
+        const unsigned zb4 = (unsigned)(wrapmod(z, G.nz) * G.nxy) * 4u;
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
             const int e = tid + q * NT;
-            if (q < KP - 1 || e < PN) preg[q] = ld32(src, poff[q]);
+            if (q < KP - 1 || e < PN) preg[q] = phi_f.ld(poff[q], zb4);
         }
     };
     auto put_phi = [&](int z) {
@@ -549,6 +581,8 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                             valid ? (unsigned)shift<1>(x, G.nx) : 0u};
     const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
                             valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
+    const Columns<S::kBytes> col(xs, ys);
+    const unsigned me4 = (ys[1] + xs[1]) * 4u;   // FP32 fields
 
     // Prologue: phi planes zb-2..zb+1, normals of planes zb-1 and zb.
     load_phi(zb - 2);
@@ -561,7 +595,20 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     comp_n(zb);
     __syncthreads();
 
-    for (int z = zb; z < ze; ++z) {
+    // The 19 population gathers of plane z are issued before the plane's
+    // barriers and normals, so they are in flight while those run (raw bits
+    // in registers; 16-bit forms decode at use).
+    using Raw = typename PopPlanes<S>::Raw;
+    auto fetch_f = [&](Raw (&r)[Q], int z) {
+        const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
+                                (unsigned)(shift<1>(z, G.nz) * G.nxy)};
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            r[l] = fsrc.template raw<l>(col.template src<l>(), zs[1 - CZ[l]] * S::kBytes);
+        });
+    };
+    // Plane z of the tile, given its gathered populations fr.
+    auto plane = [&](int z, const Raw (&fr)[Q]) {
         // Slot (z+2)&3 held plane z-2, last read by comp_n(z-1) before the
         // previous barrier.
         put_phi(z + 2);
@@ -570,11 +617,13 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
         comp_n(z + 1);
         __syncthreads();
         if (z + 1 < ze) fetch_phi(z + 3);
-        if (!valid) continue;
+        if (!valid) return;
 
+        // Wave-uniform plane offsets (elements) of z-1, z, z+1.
         const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
                                 (unsigned)(shift<1>(z, G.nz) * G.nxy)};
-        const unsigned me = zs[1] + ys[1] + xs[1];
+        const unsigned z1 = zs[1];
+        const unsigned fb = (unsigned)(G.n * 4);
         const int sm = nslot(z - 1), s0 = nslot(z), sp = nslot(z + 1);
         const int e0 = (ty + 1) * HX + (tx + 1);
         const float ncx = s_n[s0][0][e0], ncy = s_n[s0][1][e0], ncz = s_n[s0][2][e0];
@@ -590,59 +639,58 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
             if constexpr (CZ[l] != 0) az = s_n[slot][2][e];
         };
         if constexpr (S::kAnchored) {
-            cell16<MODE>(c, k16, fsrc, gsrc, fdst, gdst, me, xs, ys, zs, nbn, ncx, ncy, ncz, ind, phic);
-            continue;
-        }
-
-        float curv, Fx, Fy, Fz;
-        curvature_force(nbn, c.phys, d, ncx, ncy, ncz, ind, curv, Fx, Fy, Fz);
-
-        float fv[Q];
-        for_links([&](auto lc) {
-            constexpr int l = decltype(lc)::value;
-            fv[l] = fsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
-        });
-        const Moments m = moments(fv, c.phys, Fx, Fy, Fz);
-
-        if constexpr (MODE == COLLIDE_MACROS) {
-            st32(c.rho, me, m.rho);
-            st32(c.ux, me, m.ux);
-            st32(c.uy, me, m.uy);
-            st32(c.uz, me, m.uz);
-            st32(c.fx, me, Fx);
-            st32(c.fy, me, Fy);
-            st32(c.fz, me, Fz);
-            st32(c.nx, me, ncx);
-            st32(c.ny, me, ncy);
-            st32(c.nz, me, ncz);
+            cell16<MODE>(c, k16, fsrc, gsrc, fdst, gdst, col, me4, zs, fr, nbn, ncx, ncy, ncz, ind, phic);
         } else {
-            const FluidCell fc = fluid_cell(m, Fx, Fy, Fz);
-            const PhaseCell pc = phase_cell(c.phys, phic, m.ux, m.uy, m.uz, ncx, ncy, ncz);
-            for_links([&](auto lc) {
-                constexpr int l = decltype(lc)::value;
-                fdst.template st<l>(me, fluid_post<l>(fc, c.phys, d));
-            });
-            if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
-                // Moment form: keep what collisionPhase needs (phic is the
-                // phi buffer of this step, the next step's phi_prev).
-                st32(c.gm_u[0], me, m.ux);
-                st32(c.gm_u[1], me, m.uy);
-                st32(c.gm_u[2], me, m.uz);
-                st32(c.gm_n[0], me, ncx);
-                st32(c.gm_n[1], me, ncy);
-                st32(c.gm_n[2], me, ncz);
-                continue;
-            }
-            for_links([&](auto lc) {
-                constexpr int l = decltype(lc)::value;
-                if constexpr (MODE == COLLIDE_STEP) {
-                    gdst.template st<l>(me, phase_post<l>(pc, c.phys));
-                } else {
-                    const float gpre = gsrc.template ld<l>(zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
-                    gdst.template st<l>(me, phase_post_bgk<l>(pc, c.phys, d, gpre));
+            float curv, Fx, Fy, Fz;
+            curvature_force(nbn, c.phys, d, ncx, ncy, ncz, ind, curv, Fx, Fy, Fz);
+            const Moments m = moments(fr, c.phys, Fx, Fy, Fz);
+
+            if constexpr (MODE == COLLIDE_MACROS) {
+                Field(c.rho, fb).st(me4, z1 * 4u, m.rho);
+                Field(c.ux, fb).st(me4, z1 * 4u, m.ux);
+                Field(c.uy, fb).st(me4, z1 * 4u, m.uy);
+                Field(c.uz, fb).st(me4, z1 * 4u, m.uz);
+                Field(c.fx, fb).st(me4, z1 * 4u, Fx);
+                Field(c.fy, fb).st(me4, z1 * 4u, Fy);
+                Field(c.fz, fb).st(me4, z1 * 4u, Fz);
+                Field(c.nx, fb).st(me4, z1 * 4u, ncx);
+                Field(c.ny, fb).st(me4, z1 * 4u, ncy);
+                Field(c.nz, fb).st(me4, z1 * 4u, ncz);
+            } else {
+                const FluidCell fc = fluid_cell(m, Fx, Fy, Fz);
+                const PhaseCell pc = phase_cell(c.phys, phic, m.ux, m.uy, m.uz, ncx, ncy, ncz);
+                for_links([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    fdst.template stb<l>(col.me(), z1 * 4u, fluid_post<l>(fc, c.phys, d));
+                });
+                if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
+                    // Moment form: keep what collisionPhase needs (phic is the
+                    // phi buffer of this step, the next step's phi_prev).
+                    Field(c.gm_u[0], fb).st(me4, z1 * 4u, m.ux);
+                    Field(c.gm_u[1], fb).st(me4, z1 * 4u, m.uy);
+                    Field(c.gm_u[2], fb).st(me4, z1 * 4u, m.uz);
+                    Field(c.gm_n[0], fb).st(me4, z1 * 4u, ncx);
+                    Field(c.gm_n[1], fb).st(me4, z1 * 4u, ncy);
+                    Field(c.gm_n[2], fb).st(me4, z1 * 4u, ncz);
+                    return;
                 }
-            });
+                for_links([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    if constexpr (MODE == COLLIDE_STEP) {
+                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post<l>(pc, c.phys));
+                    } else {
+                        const float gpre = gsrc.template ldb<l>(col.template src<l>(), zs[1 - CZ[l]] * 4u);
+                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post_bgk<l>(pc, c.phys, d, gpre));
+                    }
+                });
+            }
         }
+    };
+
+    for (int z = zb; z < ze; ++z) {
+        Raw r[Q];
+        fetch_f(r, z);
+        plane(z, r);
     }
 }
 

@@ -108,6 +108,70 @@ def test_16bit_storage_against_fp64(gpu_lib, storage, case):
         assert e[k] <= RATIO[storage] * e32[k] + FLOOR[storage][k], f"{k}: {e[k]:.3e} vs FP32's {e32[k]:.3e}"
 
 
+CX = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0]
+CY = [0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1]
+CZ = [0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1]
+# Significand bits of the 16-bit forms (FP16 11, BF16 8).
+SIG_BITS = {"fp16": 11, "bf16": 8}
+
+
+@pytest.mark.parametrize("tau_g", [0.8, 1.0])
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_16bit_one_step_matches_fp64_on_a_strong_random_state(gpu_lib, storage, tau_g):
+    """One step from a random state far from equilibrium: velocities up to
+    0.15 per component, a random non-equilibrium part, and a strong
+    surface-tension force (sigma = 1 on a sharp bubble of radius 4.5, |F| up
+    to ~0.1). The 16-bit form's post-step populations equal the FP64 oracle's,
+    run from the same decoded state, to the 16-bit rounding of the stored
+    offsets (plus, on the rest link, the folded residuals of the other 18).
+    Here the cubic He-forcing terms (27/4 (c.u)^2 (c.F) w_l ~ 1e-3), the
+    velocity-force products and the regularised Pi terms are each one to two
+    orders above that bound, so a wrong coefficient or sign in the 16-bit
+    arithmetic fails this test, which multi-step comparisons near equilibrium
+    would not show."""
+    mcb = gpu_lib
+    st = mcb.STORE_FP16 if storage == "fp16" else mcb.STORE_BF16
+    shape = (20, 18, 16)                      # nx, ny, nz
+    nz, ny, nx = shape[::-1]
+    p = mcb.make_params(*shape, tau_f=0.7, tau_g=tau_g, sigma=1.0, storage=st)
+    lat = mcb.Lattice(p)
+    lat.init_bubbles([(9.6, 8.3, 7.7, 4.5)], 1.5)
+    o = oracle_for(p, prec=64)
+    w = o.weights().reshape(19, 1, 1, 1)
+    rng = np.random.default_rng(20261202)
+    u = rng.uniform(-0.15, 0.15, size=(3, nz, ny, nx))
+    rho = 1.0 + rng.uniform(-0.05, 0.05, size=(nz, ny, nx))
+    cu = sum(np.array(cc, dtype=np.float64).reshape(19, 1, 1, 1) * u[a] for a, cc in enumerate((CX, CY, CZ)))
+    feq = w * rho * (1.0 + 3.0 * cu + 4.5 * cu ** 2 - 1.5 * (u ** 2).sum(axis=0))
+    lat.write("f", feq + w * rng.normal(0.0, 0.02, size=feq.shape))
+    f0, g0 = lat.read("f"), lat.read("g")
+    o.init([(9.6, 8.3, 7.7, 4.5)], 1.5)
+    o.field("f")[:] = f0
+    o.field("g")[:] = g0
+    lat.step(1)
+    o.step(1)
+    got, ref = lat.read("f").astype(np.float64), o.field("f").astype(np.float64)
+
+    q = SIG_BITS[storage]
+    off = np.abs(ref - w)
+    bound = off * 2.0 ** (1 - q) + 2e-6       # one rounding of the offset, with margin
+    bound[0] += (off[1:] * 2.0 ** -q).sum(axis=0)
+    err = np.abs(got - ref)
+    worst = np.unravel_index(np.argmax(err / bound), err.shape)
+    print(f"{storage} tau_g {tau_g}: max |df| {err.max():.3e}, max err/bound {np.max(err / bound):.3f} "
+          f"at {worst}, offsets up to {off.max():.3e}")
+    assert np.all(err <= bound), f"link {worst[0]}: {err[worst]:.3e} > {bound[worst]:.3e}"
+
+    # The macros of the new state (the collide pass's macro mode).
+    obs = lat.observables()
+    o.macros()
+    tol = 4 * 2.0 ** -q * np.abs(off).max() * 19
+    for k in ("rho", "ux", "uy", "uz"):
+        assert np.max(np.abs(obs[k] - o.field(k))) <= tol, k
+    lat.close()
+    o.close()
+
+
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
 def test_16bit_moment_form_g_readout(gpu_lib, storage):
     """At tau_g = 1 the 16-bit forms keep g as (phi, B): its readout is
