@@ -182,6 +182,33 @@ int mcb_batch_count(const mcb_batch* b);
 size_t mcb_batch_bytes(const mcb_batch* b);
 const char* mcb_batch_last_error(const mcb_batch* b);
 
+/* ---- State images: checkpoint and restart ----
+ * No counterpart in the reference: it writes phi snapshots only
+ * (src/main.cu:184-189), which cannot restart a run (f and g are needed).
+ * An image holds a context's (or batch case's) complete lattice state at a
+ * step boundary in the form the next step reads (populations as stored, g as
+ * populations or in moment form, the state's phi, the 16-bit mass-drift
+ * state) plus its parameters and step count. Loading it into a context or
+ * case with the same parameters (device and requested path aside) continues
+ * the run bit for bit. The layout is this library's (version-checked), not
+ * the reference's; F and G in the reference layout stay available through
+ * mcb_read_field. */
+/* Bytes of the context's state image. */
+mcb_status mcb_state_size(mcb_ctx* ctx, size_t* bytes);
+/* Write the image into buf (bytes >= mcb_state_size). */
+mcb_status mcb_save_state(mcb_ctx* ctx, void* buf, size_t bytes);
+/* Restore an image into a context created with the same parameters; a
+ * context without an initial state is initialised by this call. */
+mcb_status mcb_load_state(mcb_ctx* ctx, const void* buf, size_t bytes);
+/* Parameters and step count recorded in an image (no device needed). */
+mcb_status mcb_state_info(const void* buf, size_t bytes, mcb_params* params, long long* steps);
+mcb_status mcb_batch_state_size(mcb_batch* b, int case_id, size_t* bytes);
+mcb_status mcb_batch_save_state(mcb_batch* b, int case_id, void* buf, size_t bytes);
+mcb_status mcb_batch_load_state(mcb_batch* b, int case_id, const void* buf, size_t bytes);
+/* A new case with the image's parameters and state (a preempted case
+ * resuming in another batch). */
+mcb_status mcb_batch_add_state(mcb_batch* b, const void* buf, size_t bytes, int* case_id);
+
 int mcb_abi_version(void);
 const char* mcb_last_error(const mcb_ctx* ctx);
 const char* mcb_status_string(mcb_status s);

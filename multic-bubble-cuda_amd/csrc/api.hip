@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "state.hpp"
 #include "common.hpp"
 #include "fused.hpp"
 
@@ -67,6 +68,7 @@ struct mcb_ctx {
     float* phis[2] = {nullptr, nullptr};
     float* gmu[3] = {nullptr, nullptr, nullptr};
     float* gmn[3] = {nullptr, nullptr, nullptr};
+    long long steps = 0;   // steps since the initial state (non-batch paths)
 };
 
 struct mcb_batch {
@@ -261,8 +263,39 @@ mcb_status any_step(mcb_ctx* c, int nsteps) {
     for (int t = 0; t < nsteps; ++t) {
         mcb_status st = c->path == MCB_PATH_FUSED ? fused_step(c) : staged_step(c);
         if (st != MCB_OK) return st;
+        ++c->steps;
     }
     return MCB_OK;
+}
+
+// What the next step of a non-batch context reads: staged path, every
+// reference variable (src/var.cu:13-18): with ghost-layer boundaries the
+// kernels leave the face values of phi and the macroscopic fields from one
+// iteration to the next, so they are state too; fused FP32, f of the current
+// buffer and g as populations or in moment form (the state's phi, u, n).
+std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
+    const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
+    if (c->path == MCB_PATH_STAGED) {
+        std::vector<StatePart> parts{{"f", c->sf, 19, N, N, 4, false}, {"g", c->sg, 19, N, N, 4, false}};
+        const std::pair<const char*, float*> fields[] = {
+            {"phi", c->phi},     {"rho", c->rho},       {"ux", c->ux},   {"uy", c->uy},   {"uz", c->uz},
+            {"fx", c->ffx},      {"fy", c->ffy},        {"fz", c->ffz},  {"nx", c->normx}, {"ny", c->normy},
+            {"nz", c->normz},    {"ind", c->indicator}, {"curv", c->curvature}, {"pxx", c->pxx}, {"pyy", c->pyy},
+            {"pzz", c->pzz},     {"pxy", c->pxy},       {"pxz", c->pxz}, {"pyz", c->pyz}};
+        for (const auto& f : fields) parts.push_back({f.first, f.second, 1, N, N, 4, false});
+        return parts;
+    }
+    std::vector<StatePart> parts{{"f", c->f[c->cur], 19, N, P, 4, false}};
+    if (c->gm == GM_POPULATIONS) {
+        parts.push_back({"g", c->g[c->cur], 19, N, P, 4, false});
+    } else {
+        static const char* const un[3] = {"u0", "u1", "u2"};
+        static const char* const nn[3] = {"n0", "n1", "n2"};
+        parts.push_back({"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false});
+        for (int k = 0; k < 3; ++k) parts.push_back({un[k], c->gmu[k], 1, N, N, 4, false});
+        for (int k = 0; k < 3; ++k) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
+    }
+    return parts;
 }
 
 float* scalar_field(mcb_ctx* c, int field) {
@@ -438,6 +471,7 @@ mcb_status ctx_init_bubbles_device(mcb_ctx* c, const BubbleSet& s) {
         mcb_status st = batch_status(c, c->batch->add_bubbles(c->p, s.b, s.n, s.width, &c->case_id));
         if (st != MCB_OK) return st;
         if ((st = batch_status(c, c->batch->set_tiling(c->case_id, c->tiling))) != MCB_OK) return st;
+        c->steps = 0;
         c->initialised = true;
         return MCB_OK;
     }
@@ -447,6 +481,7 @@ mcb_status ctx_init_bubbles_device(mcb_ctx* c, const BubbleSet& s) {
     MCB_HIP(c, launch_bubble_phi(c->phi, c->geo, s, c->stream));
     MCB_HIP(c, fused_init_state(c));
     MCB_HIP(c, hipStreamSynchronize(c->stream));
+    c->steps = 0;
     c->initialised = true;
     return MCB_OK;
 }
@@ -466,6 +501,7 @@ mcb_status mcb_init_phi(mcb_ctx* c, const float* phi_host, size_t n) {
         if (st != MCB_OK) return st;
         Tiling t = c->tiling;
         if ((st = batch_status(c, c->batch->set_tiling(c->case_id, t))) != MCB_OK) return st;
+        c->steps = 0;
         c->initialised = true;
         return MCB_OK;
     }
@@ -486,6 +522,7 @@ mcb_status mcb_init_phi(mcb_ctx* c, const float* phi_host, size_t n) {
         MCB_OP(c, mcb_initDist(c->rho, c->phi, c->sf, c->sg, NX, NY, NZ, &c->oc, c->stream));
     }
     MCB_HIP(c, hipStreamSynchronize(c->stream));
+    c->steps = 0;
     c->initialised = true;
     return MCB_OK;
 }
@@ -652,6 +689,7 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
         MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[2], c->stream));
         after_fused_step(c);
+        ++c->steps;
     }
     MCB_HIP(c, hipEventSynchronize(c->tev[need - 1]));
     double acc[2] = {0.0, 0.0};
@@ -861,6 +899,98 @@ int mcb_batch_count(const mcb_batch* b) { return b ? b->b->count() : 0; }
 size_t mcb_batch_bytes(const mcb_batch* b) { return b ? b->b->bytes() : 0; }
 
 const char* mcb_batch_last_error(const mcb_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+mcb_status mcb_state_size(mcb_ctx* c, size_t* bytes) {
+    mcb_status st = check_ready(c);
+    if (st != MCB_OK) return st;
+    if (!bytes) return fail(c, MCB_ERR_INVALID_ARG, "null size pointer");
+    if (c->batch) return batch_status(c, c->batch->state_size(c->case_id, bytes));
+    *bytes = state_image_bytes(ctx_state_parts(c));
+    return MCB_OK;
+}
+
+mcb_status mcb_save_state(mcb_ctx* c, void* buf, size_t bytes) {
+    mcb_status st = check_ready(c);
+    if (st != MCB_OK) return st;
+    if (c->batch) return batch_status(c, c->batch->save_state(c->case_id, buf, bytes));
+    MCB_HIP(c, hipStreamSynchronize(c->stream));
+    StateHeader h{};
+    h.params = c->p;
+    h.path = c->path;
+    h.gm = c->path == MCB_PATH_STAGED ? GM_POPULATIONS : c->gm;
+    h.steps = c->steps;
+    return state_image_save(c->stream, h, ctx_state_parts(c), buf, bytes, &c->err);
+}
+
+mcb_status mcb_load_state(mcb_ctx* c, const void* buf, size_t bytes) {
+    if (!c) return MCB_ERR_INVALID_ARG;
+    StateHeader h;
+    mcb_status st = state_image_header(buf, bytes, &h, &c->err);
+    if (st != MCB_OK) return st;
+    if (!same_state_params(h.params, c->p) || h.path != c->path)
+        return fail(c, MCB_ERR_INVALID_ARG, "state image of another lattice, parameter set or path");
+    if (h.gm < GM_POPULATIONS || h.gm > GM_INITIAL || (h.gm != GM_POPULATIONS && c->p.tau_g != 1.0f) ||
+        (c->path == MCB_PATH_STAGED && h.gm != GM_POPULATIONS))
+        return fail(c, MCB_ERR_INVALID_ARG, "state image with an invalid g form");
+    if (!c->initialised) {
+        // Any initial state gives the context its buffers (and, for 16 bits,
+        // its batch case); the image then replaces it.
+        const std::vector<float> zero((size_t)c->geo.n, 0.0f);
+        if ((st = mcb_init_phi(c, zero.data(), zero.size())) != MCB_OK) return st;
+    }
+    if ((st = check_ready(c)) != MCB_OK) return st;
+    if (c->batch) return batch_status(c, c->batch->load_state(c->case_id, buf, bytes));
+    MCB_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->path == MCB_PATH_FUSED) {
+        if (h.gm == GM_POPULATIONS && c->gm != GM_POPULATIONS) {
+            // As writing G: from here on g is kept as populations.
+            for (int b = 0; b < 2; ++b)
+                if (!c->g[b] && (st = alloc(c, &c->g[b], 19 * (size_t)c->geo.plane)) != MCB_OK) return st;
+            c->phi = c->phis[c->cur ^ 1];
+        }
+        c->gm = h.gm;
+    }
+    if ((st = state_image_load(c->stream, buf, bytes, ctx_state_parts(c), &c->err)) != MCB_OK) return st;
+    if (c->path == MCB_PATH_FUSED && c->gm != GM_POPULATIONS) c->phi = c->phis[c->cur ^ 1];
+    c->steps = h.steps;
+    return MCB_OK;
+}
+
+mcb_status mcb_state_info(const void* buf, size_t bytes, mcb_params* params, long long* steps) {
+    StateHeader h;
+    mcb_status st = state_image_header(buf, bytes, &h, nullptr);
+    if (st != MCB_OK) return st;
+    if (params) *params = h.params;
+    if (steps) *steps = h.steps;
+    return MCB_OK;
+}
+
+mcb_status mcb_batch_state_size(mcb_batch* b, int case_id, size_t* bytes) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->state_size(case_id, bytes));
+}
+
+mcb_status mcb_batch_save_state(mcb_batch* b, int case_id, void* buf, size_t bytes) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->save_state(case_id, buf, bytes));
+}
+
+mcb_status mcb_batch_load_state(mcb_batch* b, int case_id, const void* buf, size_t bytes) {
+    if (!b) return MCB_ERR_INVALID_ARG;
+    return bstat(b, b->b->load_state(case_id, buf, bytes));
+}
+
+mcb_status mcb_batch_add_state(mcb_batch* b, const void* buf, size_t bytes, int* case_id) {
+    if (!b || !case_id) return MCB_ERR_INVALID_ARG;
+    StateHeader h;
+    mcb_status st = state_image_header(buf, bytes, &h, &b->err);
+    if (st != MCB_OK) return st;
+    if (h.params.storage != b->storage) {
+        b->err = "state image of another storage form than the batch's";
+        return MCB_ERR_INVALID_ARG;
+    }
+    return bstat(b, b->b->add_state(buf, bytes, case_id));
+}
 
 // ---- device helpers
 

@@ -5,6 +5,7 @@
 #include <tuple>
 
 #include "batch.hpp"
+#include "state.hpp"
 #include "tile.hpp"
 
 namespace mcb {
@@ -593,6 +594,131 @@ mcb_status Batch::read(int id, int field, float* host, size_t n) {
     return MCB_OK;
 }
 
+mcb_status Batch::to_populations(CaseState* c) {
+    B_HIP(hipStreamSynchronize(stream_));
+    const size_t pops = 19 * (size_t)c->geo.plane * pop_bytes(c->storage);
+    for (int b = 0; b < 2; ++b) {
+        mcb_status st = dev_alloc(pops, &c->g[b]);
+        if (st != MCB_OK) return st;
+        c->allocs.emplace_back(c->g[b], pops);
+        c->bytes += pops;
+    }
+    if (c->storage != MCB_STORE_FP32) {
+        void* q = nullptr;
+        mcb_status st = dev_alloc(sizeof(MassCell), &q);
+        if (st != MCB_OK) return st;
+        c->allocs.emplace_back(q, sizeof(MassCell));
+        c->mass = static_cast<MassCell*>(q);
+        if ((st = build_own_table(c)) != MCB_OK) return st;   // partial sums
+    }
+    c->gm = GM_POPULATIONS;
+    dirty_ = true;
+    return MCB_OK;
+}
+
+// --------------------------------------------------------------- state
+
+// What the next step of a case reads (view(cur)): f of the current buffer;
+// g as populations, or its moment form (u, n in FP32; B in 16 bits); the
+// state's phi (the moment form's phi, the 16-bit g anchor); the 16-bit mass
+// target and the correction pending for the next phase pass.
+std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass2) {
+    const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
+    const unsigned eb = pop_bytes(c->storage);
+    std::vector<StatePart> parts;
+    parts.push_back({"f", c->f[c->cur], 19, N, P, eb, false});
+    if (c->gm == GM_POPULATIONS) parts.push_back({"g", c->g[c->cur], 19, N, P, eb, false});
+    parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
+    if (c->gm != GM_POPULATIONS) {
+        static const char* const un[3] = {"u0", "u1", "u2"};
+        static const char* const nn[3] = {"n0", "n1", "n2"};
+        for (int k = 0; k < 3; ++k)
+            if (c->gmu[k]) parts.push_back({un[k], c->gmu[k], 1, N, N, 4, false});
+        for (int k = 0; k < 3; ++k)
+            if (c->gmn[k]) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
+    }
+    if (c->mass) parts.push_back({"mass", mass2, 1, 2, 2, 8, true});
+    return parts;
+}
+
+mcb_status Batch::state_size(int id, size_t* bytes) {
+    CaseState* c = get(id);
+    if (!c || !bytes) return fail(MCB_ERR_INVALID_ARG, "no such case or null size pointer");
+    double m2[2];
+    *bytes = state_image_bytes(state_parts(c, m2));
+    return MCB_OK;
+}
+
+mcb_status Batch::save_state(int id, void* buf, size_t bytes) {
+    CaseState* c = get(id);
+    if (!c) return fail(MCB_ERR_INVALID_ARG, "no such case");
+    B_HIP(hipSetDevice(device_));
+    B_HIP(hipStreamSynchronize(stream_));
+    double m2[2] = {0.0, 0.0};
+    if (c->mass) {
+        MassCell m;
+        B_HIP(hipMemcpy(&m, c->mass, sizeof(m), hipMemcpyDeviceToHost));
+        m2[0] = m.target;
+        m2[1] = m.delta[c->cur];
+    }
+    StateHeader h{};
+    h.params = c->p;
+    h.path = MCB_PATH_FUSED;
+    h.gm = c->gm;
+    h.steps = c->steps;
+    return state_image_save(stream_, h, state_parts(c, m2), buf, bytes, &err_);
+}
+
+mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
+    CaseState* c = get(id);
+    if (!c) return fail(MCB_ERR_INVALID_ARG, "no such case");
+    StateHeader h;
+    mcb_status st = state_image_header(buf, bytes, &h, &err_);
+    if (st != MCB_OK) return st;
+    if (!same_state_params(h.params, c->p) || h.path != MCB_PATH_FUSED)
+        return fail(MCB_ERR_INVALID_ARG, "state image of another lattice or parameter set");
+    if (h.gm < GM_POPULATIONS || h.gm > GM_INITIAL || (h.gm != GM_POPULATIONS && c->mode != COLLIDE_STEP))
+        return fail(MCB_ERR_INVALID_ARG, "state image with an invalid g form");
+    B_HIP(hipSetDevice(device_));
+    B_HIP(hipStreamSynchronize(stream_));
+    if (h.gm == GM_POPULATIONS && c->gm != GM_POPULATIONS && (st = to_populations(c)) != MCB_OK) return st;
+    // A case that holds populations and gets a moment-form image keeps its
+    // g buffers (unused) and reads the moment fields again.
+    c->gm = h.gm;
+    double m2[2] = {0.0, 0.0};
+    if ((st = state_image_load(stream_, buf, bytes, state_parts(c, m2), &err_)) != MCB_OK) return st;
+    if (c->mass) {
+        MassCell m{m2[0], {0.0, 0.0}, 0u, 0u};
+        m.delta[c->cur] = m2[1];
+        B_HIP(hipMemcpy(c->mass, &m, sizeof(m), hipMemcpyHostToDevice));
+    }
+    c->steps = h.steps;
+    c->phi_last = c->phi[c->cur ^ 1];
+    if (macro_id_ == id) macro_id_ = 0;
+    dirty_ = true;
+    return MCB_OK;
+}
+
+mcb_status Batch::add_state(const void* buf, size_t bytes, int* id) {
+    StateHeader h;
+    mcb_status st = state_image_header(buf, bytes, &h, &err_);
+    if (st != MCB_OK) return st;
+    if (h.path != MCB_PATH_FUSED) return fail(MCB_ERR_INVALID_ARG, "state image of a staged-path context");
+    mcb_params p = h.params;
+    p.device = device_;
+    const std::vector<float> zero((size_t)p.nx * p.ny * p.nz, 0.0f);
+    int nid = 0;
+    if ((st = add(p, zero.data(), &nid)) != MCB_OK) return st;
+    if ((st = load_state(nid, buf, bytes)) != MCB_OK) {
+        const std::string keep = err_;
+        (void)remove(nid);
+        err_ = keep;
+        return st;
+    }
+    if (id) *id = nid;
+    return MCB_OK;
+}
+
 mcb_status Batch::write(int id, int field, const float* host, size_t n) {
     CaseState* c = get(id);
     if (!c || !host) return fail(MCB_ERR_INVALID_ARG, "no such case or null buffer");
@@ -606,24 +732,8 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
             // Arbitrary g populations have no moment form: the case keeps g as
             // populations from here on (16 bits: anchored to the state's phi,
             // with the mass-drift correction).
-            B_HIP(hipStreamSynchronize(stream_));
-            const size_t pops = 19 * P * pop_bytes(c->storage);
-            for (int b = 0; b < 2; ++b) {
-                mcb_status st = dev_alloc(pops, &c->g[b]);
-                if (st != MCB_OK) return st;
-                c->allocs.emplace_back(c->g[b], pops);
-                c->bytes += pops;
-            }
-            if (c->storage != MCB_STORE_FP32) {
-                void* q = nullptr;
-                mcb_status st = dev_alloc(sizeof(MassCell), &q);
-                if (st != MCB_OK) return st;
-                c->allocs.emplace_back(q, sizeof(MassCell));
-                c->mass = static_cast<MassCell*>(q);
-                if ((st = build_own_table(c)) != MCB_OK) return st;   // partial sums
-            }
-            c->gm = GM_POPULATIONS;
-            dirty_ = true;
+            mcb_status st = to_populations(c);
+            if (st != MCB_OK) return st;
         }
         float* tmp = nullptr;
         B_HIP(hipMalloc(reinterpret_cast<void**>(&tmp), 19 * P * sizeof(float)));

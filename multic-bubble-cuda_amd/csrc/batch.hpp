@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "fused.hpp"
+#include "state.hpp"
 
 namespace mcb {
 
@@ -83,6 +84,12 @@ public:
     mcb_status write(int id, int field, const float* host, size_t n);
     mcb_status reduce(int id, double* sum_phi, double* kinetic);
     mcb_status set_tiling(int id, const Tiling& t);
+    // State image of a case at a step boundary (state.hpp): size, save, load
+    // into a case of the same parameters, or a new case from an image.
+    mcb_status state_size(int id, size_t* bytes);
+    mcb_status save_state(int id, void* buf, size_t bytes);
+    mcb_status load_state(int id, const void* buf, size_t bytes);
+    mcb_status add_state(const void* buf, size_t bytes, int* id);
     mcb_status synchronize();
     // Keep up to limit bytes of freed case buffers for reuse by later cases
     // of the same sizes (0: free them at once). trim_pool frees cached blocks
@@ -107,6 +114,12 @@ private:
     mcb_status launch_step(int q);
     // 16-bit storage: reset the mass target and the pending corrections.
     mcb_status set_mass_target(CaseState* c, double target);
+    // Keep g as populations from here on (arbitrary g has no moment form):
+    // allocates the g buffers and, for 16 bits, the mass-drift state.
+    mcb_status to_populations(CaseState* c);
+    // The arrays of a case's state image; mass2 receives / supplies the
+    // 16-bit mass target and pending correction.
+    std::vector<StatePart> state_parts(CaseState* c, double* mass2);
     // One set of macro fields per batch, sized for its largest case; it holds
     // the observables of the case of the most recent macros() call.
     mcb_status ensure_macro_scratch();

@@ -166,6 +166,14 @@ PROTOTYPES = {
     "mcb_batch_count": (_i, [_vp]),
     "mcb_batch_bytes": (_sz, [_vp]),
     "mcb_batch_last_error": (_cp, [_vp]),
+    "mcb_state_size": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "mcb_save_state": (_i, [_vp, _vp, _sz]),
+    "mcb_load_state": (_i, [_vp, _vp, _sz]),
+    "mcb_state_info": (_i, [_vp, _sz, _PP, ctypes.POINTER(ctypes.c_longlong)]),
+    "mcb_batch_state_size": (_i, [_vp, _i, ctypes.POINTER(_sz)]),
+    "mcb_batch_save_state": (_i, [_vp, _i, _vp, _sz]),
+    "mcb_batch_load_state": (_i, [_vp, _i, _vp, _sz]),
+    "mcb_batch_add_state": (_i, [_vp, _vp, _sz, ctypes.POINTER(_i)]),
     "mcb_serve_run": (_i, [ctypes.POINTER(ServeConfig), ctypes.POINTER(Request), _i, ctypes.POINTER(RequestResult),
                            ctypes.POINTER(ServeStats)]),
     "mcb_serve_trace": (_i, [ctypes.c_uint64, _i, _d, _d, ctypes.POINTER(Request)]),
@@ -372,6 +380,32 @@ class Lattice:
         self.compute_macros()
         return {k: self.read(k) for k in ("phi", "rho", "ux", "uy", "uz")}
 
+    def save_state(self):
+        """The complete lattice state as bytes (mcb_save_state): loading it into
+        a Lattice with the same parameters continues the run bit for bit."""
+        n = ctypes.c_size_t()
+        self._chk(lib().mcb_state_size(self._h, ctypes.byref(n)), "mcb_state_size")
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(lib().mcb_save_state(self._h, buf, n.value), "mcb_save_state")
+        return buf.raw
+
+    def load_state(self, image):
+        buf = _image_buffer(image)
+        self._chk(lib().mcb_load_state(self._h, buf, len(buf)), "mcb_load_state")
+
+
+def _image_buffer(image):
+    b = bytes(image)
+    return (ctypes.c_char * len(b)).from_buffer_copy(b)
+
+
+def state_info(image):
+    """(Params, steps) recorded in a state image."""
+    buf = _image_buffer(image)
+    p, steps = Params(), ctypes.c_longlong()
+    check(lib().mcb_state_info(buf, len(buf), ctypes.byref(p), ctypes.byref(steps)), what="mcb_state_info")
+    return p, steps.value
+
 
 def autotune(params):
     """Time every tile shape on a scratch lattice; returns (tx, ty, zc, ms_per_step)."""
@@ -469,6 +503,27 @@ class Batch:
         if st not in (OK, ERR_NONFINITE):
             self._chk(st, "mcb_batch_reduce")
         return s.value, k.value, st == OK
+
+    def save_state(self, cid):
+        """State image of one case (mcb_batch_save_state)."""
+        n = ctypes.c_size_t()
+        self._chk(lib().mcb_batch_state_size(self._h, cid, ctypes.byref(n)), "mcb_batch_state_size")
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(lib().mcb_batch_save_state(self._h, cid, buf, n.value), "mcb_batch_save_state")
+        return buf.raw
+
+    def load_state(self, cid, image):
+        buf = _image_buffer(image)
+        self._chk(lib().mcb_batch_load_state(self._h, cid, buf, len(buf)), "mcb_batch_load_state")
+
+    def add_state(self, image):
+        """A new case resuming from a state image; returns its id."""
+        buf = _image_buffer(image)
+        cid = ctypes.c_int()
+        self._chk(lib().mcb_batch_add_state(self._h, buf, len(buf), ctypes.byref(cid)), "mcb_batch_add_state")
+        p, _ = state_info(image)
+        self.shapes[cid.value] = (p.nz, p.ny, p.nx)
+        return cid.value
 
     def case_steps(self, cid):
         v = ctypes.c_longlong()

@@ -204,3 +204,19 @@ def test_driver_usage_without_arguments():
     assert r.returncode == 1 and "Usage" in r.stderr
     r = subprocess.run([drv, "D3Q27", "001"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "D3Q19" in r.stderr
+
+
+@pytest.mark.parametrize("blob", [b"", b"MCBSTATE", b"x" * 256])
+def test_state_info_rejects_what_is_not_a_state_image(blob):
+    """mcb_state_info reads an image's header without a device; anything that
+    is not a whole image of this version is refused with a status."""
+    with pytest.raises(mcb.McbError) as e:
+        mcb.state_info(blob)
+    assert e.value.status == mcb.ERR_INVALID_ARG
+
+
+def test_state_calls_without_a_device_report_status():
+    n = ctypes.c_size_t()
+    assert mcb.lib().mcb_state_size(None, ctypes.byref(n)) == mcb.ERR_INVALID_ARG
+    assert mcb.lib().mcb_load_state(None, None, 0) == mcb.ERR_INVALID_ARG
+    assert mcb.lib().mcb_batch_add_state(None, None, 0, None) == mcb.ERR_INVALID_ARG
