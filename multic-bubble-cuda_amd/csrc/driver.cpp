@@ -18,11 +18,17 @@
 //   --path auto|fused|staged            kernel path
 //   --extra-fields      also write rho, ux, uy, uz (names without "phi")
 //   --device D          HIP device ordinal
+//   --load-state FILE   resume from a state image (mcb_save_state) of the
+//                       same case: iterations continue after those it holds
+//   --save-state FILE   write the state image at the end of the run
+//   --state-every K     also write it after every K iterations (replacing
+//                       FILE through a temporary file and a rename)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <filesystem>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -38,6 +44,34 @@ int die(const char* what, mcb_status st, mcb_ctx* ctx) {
     return 2;
 }
 
+bool read_file(const std::string& path, std::vector<char>* out) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    in.seekg(0, std::ios::end);
+    const std::streamoff n = in.tellg();
+    if (n < 0) return false;
+    out->resize((size_t)n);
+    in.seekg(0);
+    return (bool)in.read(out->data(), n);
+}
+
+// The context's state image into path, through path.tmp and a rename.
+mcb_status write_state(mcb_ctx* ctx, const std::string& path) {
+    size_t n = 0;
+    mcb_status st = mcb_state_size(ctx, &n);
+    if (st != MCB_OK) return st;
+    std::vector<char> buf(n);
+    if ((st = mcb_save_state(ctx, buf.data(), n)) != MCB_OK) return st;
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream out(tmp, std::ios::binary | std::ios::trunc);
+        if (!out || !out.write(buf.data(), (std::streamsize)n)) return MCB_ERR_IO;
+    }
+    std::error_code ec;
+    std::filesystem::rename(tmp, path, ec);
+    return ec ? MCB_ERR_IO : MCB_OK;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -49,6 +83,8 @@ int main(int argc, char** argv) {
     float tau_f = -1, tau_g = -1, sigma = -1;
     bool extra = false;
     int device = 0;
+    std::string load_state, save_state;
+    long long state_every = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto val = [&](const char* nm) -> std::string {
@@ -69,6 +105,9 @@ int main(int argc, char** argv) {
         else if (a == "--path") path = val("--path");
         else if (a == "--extra-fields") extra = true;
         else if (a == "--device") device = std::atoi(val("--device").c_str());
+        else if (a == "--load-state") load_state = val("--load-state");
+        else if (a == "--save-state") save_state = val("--save-state");
+        else if (a == "--state-every") state_every = std::atoll(val("--state-every").c_str());
         else if (a.rfind("--", 0) == 0) {
             std::cerr << "mcb: unknown option " << a << std::endl;
             return 1;
@@ -104,9 +143,27 @@ int main(int argc, char** argv) {
         return 2;
     }
 
+    if (state_every > 0 && save_state.empty()) {
+        std::cerr << "mcb: --state-every needs --save-state" << std::endl;
+        return 1;
+    }
     mcb_ctx* ctx = nullptr;
     if ((st = mcb_create(&cs.params, nullptr, &ctx)) != MCB_OK) return die("mcb_create", st, nullptr);
-    if ((st = mcb_init_bubbles(ctx, cs.n_bubbles, cs.bubbles, cs.width)) != MCB_OK) return die("init", st, ctx);
+    // Iterations already taken: 0, or those a loaded state image holds.
+    long long first = 0;
+    if (!load_state.empty()) {
+        std::vector<char> img;
+        if (!read_file(load_state, &img)) {
+            std::cerr << "mcb: cannot read " << load_state << std::endl;
+            return 2;
+        }
+        if ((st = mcb_load_state(ctx, img.data(), img.size())) != MCB_OK) return die("load state", st, ctx);
+        if ((st = mcb_state_info(img.data(), img.size(), nullptr, &first)) != MCB_OK)
+            return die("state info", st, ctx);
+        std::cout << "Resumed from " << load_state << " after " << first << " iterations" << std::endl;
+    } else if ((st = mcb_init_bubbles(ctx, cs.n_bubbles, cs.bubbles, cs.width)) != MCB_OK) {
+        return die("init", st, ctx);
+    }
 
     const int NX = cs.params.nx, NY = cs.params.ny, NZ = cs.params.nz;
     const size_t cells = (size_t)NX * NY * NZ;
@@ -133,11 +190,13 @@ int main(int argc, char** argv) {
     };
 
     // STEP = 0..NSTEPS inclusive; after iteration STEP, save if STEP % save == 0.
+    // A state image written after iteration STEP holds STEP + 1 iterations.
     double device_ms = 0.0;
     long long done = 0;
-    for (long long step = 0; step <= cs.nsteps;) {
+    for (long long step = first; step <= cs.nsteps;) {
         long long next_save = ((step + cs.save_every - 1) / cs.save_every) * cs.save_every;
         long long last = std::min<long long>(next_save, cs.nsteps);
+        if (state_every > 0) last = std::min(last, ((step + state_every) / state_every) * state_every - 1);
         int chunk = (int)(last - step + 1);
         double ms = 0.0;
         if ((st = mcb_step_timed(ctx, chunk, &ms)) != MCB_OK) return die("step", st, ctx);
@@ -148,6 +207,13 @@ int main(int argc, char** argv) {
             if ((st = save((int)last)) != MCB_OK) return die("save", st, ctx);
             std::cout << "Step " << last << " of " << cs.nsteps << ": data saved in " << sim_dir << std::endl;
         }
+        if (state_every > 0 && (last + 1) % state_every == 0 && (st = write_state(ctx, save_state)) != MCB_OK)
+            return die("save state", st, ctx);
+    }
+    if (!save_state.empty()) {
+        if ((st = write_state(ctx, save_state)) != MCB_OK) return die("save state", st, ctx);
+        std::cout << "State after " << std::max(first, cs.nsteps + 1LL) << " iterations saved in " << save_state
+                  << std::endl;
     }
     mcb_destroy(ctx);
 
@@ -156,7 +222,7 @@ int main(int argc, char** argv) {
     const double mlups_dev = device_ms > 0 ? (double)cells * done / (device_ms * 1e3) : 0.0;
     std::cout << "\n// =============================================== //\n"
               << "     Total wall time: " << wall << " s\n"
-              << "     Iterations: " << done << " (NSTEPS+1)\n"
+              << "     Iterations: " << done << (first > 0 ? " (after the loaded state's)" : " (NSTEPS+1)") << "\n"
               << "     Device time: " << device_ms / 1e3 << " s, " << mlups_dev << " MLUPS\n"
               << "// =============================================== //\n"
               << std::endl;

@@ -127,3 +127,31 @@ def test_cfg1_geometry_bitexact_short(gpu_lib):
     for k in ("phi", "rho", "ux", "uy", "uz"):
         assert np.array_equal(got[k], o.field(k)), k
     lat.close()
+
+
+@pytest.mark.parametrize("tau_g", ["0.8", "1"])
+def test_driver_resumes_from_a_state_image(gpu_lib, tmp_path, tau_g):
+    """--save-state after 20 iterations, then --load-state in a second run:
+    the phi files of the resumed run are the bytes of one run straight
+    through."""
+    mcb = gpu_lib
+    common = ["--preset", "cfg0", "--tau-g", tau_g, "--save-every", "20"]
+    straight, resumed = str(tmp_path / "a") + "/", str(tmp_path / "b") + "/"
+    image = str(tmp_path / "state.img")
+    runs = [
+        [mcb.DRIVER_PATH, "D3Q19", "s", *common, "--steps", "40", "--root", straight],
+        [mcb.DRIVER_PATH, "D3Q19", "s", *common, "--steps", "19", "--save-state", image, "--root", resumed],
+        [mcb.DRIVER_PATH, "D3Q19", "s", *common, "--steps", "40", "--load-state", image, "--root", resumed],
+    ]
+    for cmd in runs:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+    assert "Resumed from" in r.stdout
+    for step in (20, 40):
+        a = open(postfmt.macr_file(straight + "bin/D3Q19/s/", "s", "phi", step), "rb").read()
+        b = open(postfmt.macr_file(resumed + "bin/D3Q19/s/", "s", "phi", step), "rb").read()
+        assert a == b, f"step {step}"
+    # An image of another case is refused with a status, not a crash.
+    r = subprocess.run([mcb.DRIVER_PATH, "D3Q19", "t", "--preset", "cfg1", "--steps", "2", "--load-state", image,
+                        "--root", resumed], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "load state" in r.stderr
