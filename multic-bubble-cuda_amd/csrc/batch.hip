@@ -45,7 +45,7 @@ size_t case_bytes(const mcb_params& p) {
     const bool moments = p.tau_g == 1.0f;
     size_t b = (moments ? 2 : 4) * 19 * P * pop_bytes(p.storage);          // f (and g) A/B
     b += (moments || p.storage != MCB_STORE_FP32 ? 2 : 1) * P * sizeof(float);   // phi
-    if (moments) b += (p.storage == MCB_STORE_FP32 ? 6 : 3) * P * sizeof(float);   // u, n or B
+    if (moments) b += p.storage == MCB_STORE_FP32 ? 6 * P * sizeof(float) : 3 * P * 2;   // u, n (FP32) or B (FP16)
     return b;
 }
 
@@ -252,10 +252,8 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     };
     mcb_status st;
     const size_t pops = 19 * (size_t)P * pop_bytes(p.storage);
-    // FP32 at tau_g = 1 keeps g in moment form (phi, u, n of the last
-    // collision) instead of two sets of 19 planes.
-    // At tau_g = 1 g is kept in moment form (phi, u, n of the last collision,
-    // FP32) instead of two sets of 19 planes, for every storage form.
+    // At tau_g = 1 g is kept in moment form instead of two sets of 19
+    // planes, for every storage form.
     const bool moments = c->mode == COLLIDE_STEP;
     for (int b = 0; b < 2; ++b) {
         if ((st = alloc(pops, &c->f[b])) != MCB_OK) return st;
@@ -271,10 +269,10 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     }
     if (moments) {
         // FP32: u and n (collisionPhase's inputs, bit-exact); 16 bits: the
-        // vector B = 3 phi u + phi_n n (phase_tile_ab).
-        const int nfields = p.storage == MCB_STORE_FP32 ? 6 : 3;
-        for (int k = 0; k < nfields; ++k) {
-            if ((st = alloc((size_t)P * sizeof(float), &q)) != MCB_OK) return st;
+        // vector B = 3 phi u + phi_n n in FP16 (phase_tile_ab).
+        const bool fp32 = p.storage == MCB_STORE_FP32;
+        for (int k = 0; k < (fp32 ? 6 : 3); ++k) {
+            if ((st = alloc((size_t)P * (fp32 ? sizeof(float) : 2), &q)) != MCB_OK) return st;
             (k < 3 ? c->gmu[k] : c->gmn[k - 3]) = static_cast<float*>(q);
         }
         c->gm = GM_INITIAL;
@@ -633,7 +631,7 @@ std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass2) {
         static const char* const un[3] = {"u0", "u1", "u2"};
         static const char* const nn[3] = {"n0", "n1", "n2"};
         for (int k = 0; k < 3; ++k)
-            if (c->gmu[k]) parts.push_back({un[k], c->gmu[k], 1, N, N, 4, false});
+            if (c->gmu[k]) parts.push_back({un[k], c->gmu[k], 1, N, N, c->storage == MCB_STORE_FP32 ? 4u : 2u, false});
         for (int k = 0; k < 3; ++k)
             if (c->gmn[k]) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
     }

@@ -61,7 +61,8 @@ struct CaseView {
                             // moment form: phi of the previous step's collision
     // Moment form of g (FP32, tau_g = 1): g*_l(y) is collisionPhase of
     // (phi, u, n) at y, so the state keeps u and n (gm_u, gm_n) instead of
-    // 19 g planes and the phase pass recomputes g*_l(x - c_l).
+    // 19 g planes and the phase pass recomputes g*_l(x - c_l). 16-bit forms:
+    // gm_u holds B = 3 phi u + phi_n n as FP16 (2 bytes per cell; ld_b).
     float* gm_u[3];
     float* gm_n[3];
     int gm;                 // GM_POPULATIONS, GM_MOMENTS or GM_INITIAL

@@ -113,9 +113,9 @@ __global__ __launch_bounds__(PBX * PBY) void k_ab_to_ref(CaseView c, float* __re
         } else {
             const unsigned s = o.template nb<l, -1>();
             float a = ld32(c.phi_prev, s);
-            if constexpr (CX[l] != 0) a = CX[l] > 0 ? a + ld32(c.gm_u[0], s) : a - ld32(c.gm_u[0], s);
-            if constexpr (CY[l] != 0) a = CY[l] > 0 ? a + ld32(c.gm_u[1], s) : a - ld32(c.gm_u[1], s);
-            if constexpr (CZ[l] != 0) a = CZ[l] > 0 ? a + ld32(c.gm_u[2], s) : a - ld32(c.gm_u[2], s);
+            if constexpr (CX[l] != 0) a = CX[l] > 0 ? a + ld_b(c.gm_u[0], s) : a - ld_b(c.gm_u[0], s);
+            if constexpr (CY[l] != 0) a = CY[l] > 0 ? a + ld_b(c.gm_u[1], s) : a - ld_b(c.gm_u[1], s);
+            if constexpr (CZ[l] != 0) a = CZ[l] > 0 ? a + ld_b(c.gm_u[2], s) : a - ld_b(c.gm_u[2], s);
             v = c.phys.w[WC[l]] * a;
         }
         dst[(size_t)l * G.plane + me] = v;

@@ -133,6 +133,29 @@ template <class S> struct PopPlanes {
     }
 };
 
+// B = 3 phi u + phi_n n, the vector of the 16-bit forms' (phi, B) moment
+// form of g, is kept in FP16 scaled by 2^10: an exact scale, |B| (a few
+// tenths at most) stays far below FP16's largest value / 2^10, and values
+// down to ~6e-8 stay normal. B only moves phi between cells (its terms cancel
+// in the sum of phi), so its rounding does not touch mass conservation.
+constexpr float kBScale = 1024.0f, kBUnscale = 0x1p-10f;
+__device__ __forceinline__ unsigned short enc_b(float v) {
+    return __builtin_bit_cast(unsigned short, (_Float16)(v * kBScale));
+}
+__device__ __forceinline__ float ld_b(const float* field, unsigned elem) {
+    typedef const unsigned short __attribute__((address_space(1)))* gp16;
+    return (float)__builtin_bit_cast(_Float16, ((gp16)field)[elem]) * kBUnscale;
+}
+// 16-bit stores into one field behind a buffer resource (as Field).
+struct Field16 {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Field16(const void* base, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferConfigWord)) {}
+    __device__ __forceinline__ void st(unsigned vb, unsigned sb, unsigned short v) const {
+        __builtin_amdgcn_raw_buffer_store_b16(v, r, vb, sb, 0);
+    }
+};
+
 // Byte offsets within a plane of the 9 columns (x - c_x, y - c_y) that a
 // cell's pull-form gathers read, for elements of EB bytes. They do not depend
 // on z, so a tile computes them once; per plane only the wave-uniform plane
@@ -343,9 +366,9 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
             if (q < KH - 1 || e < HN) {
                 const unsigned o = base + hoff[q];
                 reg[q][0] = ld32(c.phi_prev, o);
-                reg[q][1] = ld32(c.gm_u[0], o);
-                reg[q][2] = ld32(c.gm_u[1], o);
-                reg[q][3] = ld32(c.gm_u[2], o);
+                reg[q][1] = ld_b(c.gm_u[0], o);
+                reg[q][2] = ld_b(c.gm_u[1], o);
+                reg[q][3] = ld_b(c.gm_u[2], o);
             }
         }
     };
@@ -446,9 +469,10 @@ __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k,
         if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
             // Moment form of g (tau_g = 1): B = 3 phi u + phi_n n (phase_tile_ab).
             const float p3 = 3.0f * phic;
-            Field(c.gm_u[0], fb).st(me4, z1 * 4u, p3 * ux + phin * ncx);
-            Field(c.gm_u[1], fb).st(me4, z1 * 4u, p3 * uy + phin * ncy);
-            Field(c.gm_u[2], fb).st(me4, z1 * 4u, p3 * uz + phin * ncz);
+            const unsigned fb2 = (unsigned)(c.geo.n * 2);
+            Field16(c.gm_u[0], fb2).st(col.me(), z1 * 2u, enc_b(p3 * ux + phin * ncx));
+            Field16(c.gm_u[1], fb2).st(col.me(), z1 * 2u, enc_b(p3 * uy + phin * ncy));
+            Field16(c.gm_u[2], fb2).st(col.me(), z1 * 2u, enc_b(p3 * uz + phin * ncz));
             return;
         }
         // g offset of link l after the phase collision, from its odd part.

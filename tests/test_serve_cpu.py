@@ -82,10 +82,10 @@ def test_case_bytes(mcb):
     p.storage = mcb.STORE_FP16
     assert mcb.case_bytes(p) == 4 * 19 * n * 2 + 2 * n * 4
     # tau_g = 1: f populations and g in moment form: two phi, and u, n (FP32)
-    # or B = 3 phi u + phi_n n (16 bits)
+    # or B = 3 phi u + phi_n n (FP16, 16-bit forms)
     p = mcb.make_params(64, 64, 64, tau_g=1.0)
     assert mcb.case_bytes(p) == 2 * 19 * n * 4 + 8 * n * 4
     p.storage = mcb.STORE_FP16
-    assert mcb.case_bytes(p) == 2 * 19 * n * 2 + 5 * n * 4
+    assert mcb.case_bytes(p) == 2 * 19 * n * 2 + 2 * n * 4 + 3 * n * 2
     p.tau_f = 0.4
     assert mcb.case_bytes(p) == 0   # invalid parameters
