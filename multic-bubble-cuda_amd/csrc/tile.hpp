@@ -213,25 +213,25 @@ __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>
 }
 
 // Moment-form phase pass for the TX x TY column (x0, y0) over planes
-// [zb, ze): the state of each source cell is staged once per plane into a
-// 3-plane LDS ring with a 1-cell halo, as the factors collisionPhase builds
-// from it (w1 phi, w2 phi, phi_norm = sharp_c phi (1 - phi), u, n), and each
-// cell then sums g*_l(x - c_l) = (w_l phi)(1 + c.u/cs^2) + w_l (phi_norm c.n)
-// over l in link order. Every factor is formed by the same operations as in
-// collisionPhase (lbm.cu:319-326), so the sum has the stored populations'
-// bits. Must be called by all TX*TY threads (it contains barriers).
+// [zb, ze). For destination plane z, every halo cell y of the tile (a 1-cell
+// ring around it) computes the 19 values g*_l(y) that links l carry from it
+// into plane z: from its plane z - c_z(l), with collisionPhase's own
+// expressions (phase_post, lbm.cu:319-326), and writes them to LDS. Each
+// (source cell, link) value is computed exactly once; each destination then
+// reads its 19 values and sums them in link order, so the result carries the
+// stored populations' bits. Source fields (phi, u, n) stay in registers: a
+// 4-plane window (z-1, z, z+1 and z+2 in flight), with compile-time slots
+// (the z loop is unrolled by 4). Must be called by all TX*TY threads (it
+// contains barriers).
 template <int TX, int TY>
 __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0, int zb, int ze) {
     constexpr int NT = TX * TY;
     constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
     constexpr int KH = (HN + NT - 1) / NT;
-    enum { W1PHI, W2PHI, PHIN, UX, UY, UZ, NX, NY, NZ, NF };
-    __shared__ float s_m[3][NF][HN];
+    __shared__ float s_g[HN * Q];   // [halo cell][link]: g*_l of the source cell
 
     const Geo& G = c.geo;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const float w0 = c.phys.w[0], w1 = c.phys.w[1], w2 = c.phys.w[2], sc = c.phys.sharp_c;
-
     unsigned hoff[KH];
 #pragma unroll
     for (int q = 0; q < KH; ++q) {
@@ -243,89 +243,84 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
         }
         hoff[q] = off;
     }
-    // Staging is software-pipelined: fetch() loads a plane into registers,
-    // put() writes it to its LDS slot one iteration later, so the global
-    // loads of plane z+2 are in flight while plane z is summed.
-    float reg[KH][7];
-    auto fetch = [&](int z) {
+    // win[slot][q]: phi, ux, uy, uz, nx, ny, nz of halo cell q on one plane.
+    float win[4][KH][7];
+    auto fetch = [&](float (&r)[KH][7], int z) {
         const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
 #pragma unroll
         for (int q = 0; q < KH; ++q) {
-            const int e = tid + q * NT;
-            if (q < KH - 1 || e < HN) {
-                const unsigned o = base + hoff[q];
-                reg[q][0] = ld32(c.phi_prev, o);
-                reg[q][1] = ld32(c.gm_u[0], o);
-                reg[q][2] = ld32(c.gm_u[1], o);
-                reg[q][3] = ld32(c.gm_u[2], o);
-                reg[q][4] = ld32(c.gm_n[0], o);
-                reg[q][5] = ld32(c.gm_n[1], o);
-                reg[q][6] = ld32(c.gm_n[2], o);
-            }
+            const unsigned o = base + hoff[q];
+            r[q][0] = ld32(c.phi_prev, o);
+            r[q][1] = ld32(c.gm_u[0], o);
+            r[q][2] = ld32(c.gm_u[1], o);
+            r[q][3] = ld32(c.gm_u[2], o);
+            r[q][4] = ld32(c.gm_n[0], o);
+            r[q][5] = ld32(c.gm_n[1], o);
+            r[q][6] = ld32(c.gm_n[2], o);
         }
     };
-    auto put = [&](int z) {
-        float (*dst)[HN] = s_m[wrapmod(z, 3)];
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    const int e0 = (ty + 1) * HX + (tx + 1);
+    // Destination plane z with the window slots of planes z-1, z, z+1 (PM,
+    // P0, PP) and the slot that receives plane z+2 (PN).
+    auto iter = [&](int z, auto pm, auto p0, auto pp, auto pn) {
+        constexpr int PM = decltype(pm)::value, P0 = decltype(p0)::value, PP = decltype(pp)::value;
+        constexpr int PN = decltype(pn)::value;
+        if (z + 1 < ze) fetch(win[PN], z + 2);   // read by the next plane
 #pragma unroll
         for (int q = 0; q < KH; ++q) {
             const int e = tid + q * NT;
             if (q < KH - 1 || e < HN) {
-                const float phi = reg[q][0];
-                dst[W1PHI][e] = w1 * phi;
-                dst[W2PHI][e] = w2 * phi;
-                dst[PHIN][e] = sc * phi * (1.0f - phi);
-                dst[UX][e] = reg[q][1];
-                dst[UY][e] = reg[q][2];
-                dst[UZ][e] = reg[q][3];
-                dst[NX][e] = reg[q][4];
-                dst[NY][e] = reg[q][5];
-                dst[NZ][e] = reg[q][6];
+                auto cell = [&](const float (&f)[7]) {
+                    return phase_cell(c.phys, f[0], f[1], f[2], f[3], f[4], f[5], f[6]);
+                };
+                const PhaseCell cm = cell(win[PM][q]), c0 = cell(win[P0][q]), cp = cell(win[PP][q]);
+                float* dst = s_g + e * Q;
+                for_links([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    if constexpr (l == 0 || CZ[l] != 0) {
+                        dst[l] = phase_post<l>(CZ[l] > 0 ? cm : (CZ[l] < 0 ? cp : c0), c.phys);
+                    } else if constexpr (l % 2 == 1) {
+                        // In-plane pair (l, l+1), both from this plane's cell:
+                        // c_{l+1} = -c_l, so the dot products of link l+1 are
+                        // the exact negatives of link l's (IEEE rounding is
+                        // sign-symmetric) and phase_post<l+1> equals this.
+                        static_assert(CX[l + 1] == -CX[l] && CY[l + 1] == -CY[l] && CZ[l + 1] == 0, "pair");
+                        const float w = c.phys.w[WC[l]];
+                        const float ud = cdot<l>(c0.ux, c0.uy, c0.uz) * INV_CSSQ;
+                        const float wa = w * (c0.phi_norm * cdot<l>(c0.nx, c0.ny, c0.nz));
+                        const float wphi = w * c0.phi;
+                        dst[l] = wphi * (1.0f + ud) + wa;
+                        dst[l + 1] = wphi * (1.0f + -ud) + -wa;
+                    }
+                });
             }
         }
-    };
-
-    const int x = x0 + tx, y = y0 + ty;
-    const bool valid = x < G.nx && y < G.ny;
-    const int e0 = (ty + 1) * HX + (tx + 1);
-    fetch(zb - 1);
-    put(zb - 1);
-    fetch(zb);
-    put(zb);
-    fetch(zb + 1);
-    for (int z = zb; z < ze; ++z) {
-        // The slot of plane z+1 held plane z-2, last read in iteration z-1
-        // before its closing barrier.
-        put(z + 1);
         __syncthreads();
-        if (z + 1 < ze) fetch(z + 2);
         if (valid) {
-            const unsigned me = (unsigned)(z * G.nxy + y * G.nx + x);
             float sum = 0.0f;
             for_links([&](auto lc) {
                 constexpr int l = decltype(lc)::value;
-                if constexpr (l == 0) {
-                    // phase_post<0> of the cell itself: c_0 = 0.
-                    const float phi = ld32(c.phi_prev, me);
-                    const float phin = sc * phi * (1.0f - phi);
-                    const float geq = w0 * phi * (1.0f + cdot<0>(0.0f, 0.0f, 0.0f) * INV_CSSQ);
-                    sum += geq + w0 * (phin * cdot<0>(0.0f, 0.0f, 0.0f));
-                } else {
-                    const float(*p)[HN] = s_m[wrapmod(z - CZ[l], 3)];
-                    const int e = e0 - CY[l] * HX - CX[l];
-                    float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-                    if constexpr (CX[l] != 0) { ax = p[UX][e]; bx = p[NX][e]; }
-                    if constexpr (CY[l] != 0) { ay = p[UY][e]; by = p[NY][e]; }
-                    if constexpr (CZ[l] != 0) { az = p[UZ][e]; bz = p[NZ][e]; }
-                    const float udotc = cdot<l>(ax, ay, az) * INV_CSSQ;
-                    const float wphi = WC[l] == 1 ? p[W1PHI][e] : p[W2PHI][e];
-                    const float w = WC[l] == 1 ? w1 : w2;
-                    const float geq = wphi * (1.0f + udotc);
-                    sum += geq + w * (p[PHIN][e] * cdot<l>(bx, by, bz));
-                }
+                sum += s_g[(e0 - CY[l] * HX - CX[l]) * Q + l];
             });
-            st32(c.phi, me, sum);
+            st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), sum);
         }
-        __syncthreads();
+        __syncthreads();   // s_g is rewritten by the next plane
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    // Slot of plane p: (p - (zb - 1)) mod 4.
+    fetch(win[0], zb - 1);
+    fetch(win[1], zb);
+    fetch(win[2], zb + 1);
+    for (int z = zb; z < ze; z += 4) {
+        iter(z, I0{}, I1{}, I2{}, I3{});
+        if (z + 1 < ze) iter(z + 1, I1{}, I2{}, I3{}, I0{});
+        if (z + 2 < ze) iter(z + 2, I2{}, I3{}, I0{}, I1{});
+        if (z + 3 < ze) iter(z + 3, I3{}, I0{}, I1{}, I2{});
     }
 }
 
