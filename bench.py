@@ -184,7 +184,6 @@ def main():
         st = mcb.STORE_FP16 if args.form16 == "fp16" else mcb.STORE_BF16
         print(json.dumps({args.form16: storage_form(mcb, case, st, args.steps, args.warmup)}), flush=True)
         return
-        return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
     cells = p.nx * p.ny * p.nz

@@ -212,90 +212,197 @@ __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>
     return sum;
 }
 
+// Links that a halo-ring cell sends into the tile, by side of the ring:
+// 0 = row y0-1 (c_y = +1), 1 = row y0+TY (c_y = -1), 2 = column x0-1
+// (c_x = +1, corners included), 3 = column x0+TX (c_x = -1). With p = s e_a
+// (a the side's axis, s its sign) and b the other in-plane axis, the five
+// links are p, p + e_b, p - e_b, p + e_z, p - e_z, in that order.
+constexpr int RING_LINK[4][5] = {{3, 7, 14, 11, 17}, {4, 13, 8, 18, 12}, {1, 7, 13, 9, 15}, {2, 14, 8, 16, 10}};
+constexpr bool ring_links_ok() {
+    for (int side = 0; side < 4; ++side) {
+        const int s = side % 2 == 0 ? 1 : -1;
+        const int pa[3] = {side < 2 ? 0 : s, side < 2 ? s : 0, 0};
+        const int eb[3] = {side < 2 ? 1 : 0, side < 2 ? 0 : 1, 0};
+        const int want[5][3] = {{pa[0], pa[1], 0},
+                                {pa[0] + eb[0], pa[1] + eb[1], 0},
+                                {pa[0] - eb[0], pa[1] - eb[1], 0},
+                                {pa[0], pa[1], 1},
+                                {pa[0], pa[1], -1}};
+        for (int k = 0; k < 5; ++k) {
+            const int l = RING_LINK[side][k];
+            if (CX[l] != want[k][0] || CY[l] != want[k][1] || CZ[l] != want[k][2]) return false;
+        }
+    }
+    return true;
+}
+static_assert(ring_links_ok(), "ring link table");
+
+// One link value of collisionPhase at tau_g = 1 (lbm.cu:323-326, phase_post)
+// from the link's dot products cu = c.u and cn = c.n, with wphi = w phi formed
+// once per weight class. Same operations in the same order as phase_post.
+__device__ __forceinline__ float phase_link(float w, float wphi, float phi_norm, float cu, float cn) {
+    return wphi * (1.0f + cu * INV_CSSQ) + w * (phi_norm * cn);
+}
+
+// Cells of the 1-cell ring around a TX x TY tile that send links into it
+// (the four corners only along the columns' sides).
+template <int TX, int TY> constexpr int gm_ring_cells() { return 2 * TX + 2 * (TY + 2); }
+
 // Moment-form phase pass for the TX x TY column (x0, y0) over planes
-// [zb, ze). For destination plane z, every halo cell y of the tile (a 1-cell
-// ring around it) computes the 19 values g*_l(y) that links l carry from it
-// into plane z: from its plane z - c_z(l), with collisionPhase's own
-// expressions (phase_post, lbm.cu:319-326), and writes them to LDS. Each
-// (source cell, link) value is computed exactly once; each destination then
-// reads its 19 values and sums them in link order, so the result carries the
-// stored populations' bits. Source fields (phi, u, n) stay in registers: a
-// 4-plane window (z-1, z, z+1 and z+2 in flight), with compile-time slots
-// (the z loop is unrolled by 4). Must be called by all TX*TY threads (it
-// contains barriers).
+// [zb, ze). For destination plane z, every cell y of the tile and of the
+// 1-cell ring around it writes to LDS the values g*_l(y) that links l carry
+// from it into plane z (from its plane z - c_z(l)), computed with
+// collisionPhase's own operations (phase_post, lbm.cu:319-326); each
+// destination then reads its 19 values and sums them in link order, so the
+// result carries the stored populations' bits.
+//   Each value is computed once. In-plane opposite links l, l' (c_l' = -c_l)
+// of a cell share their work: their dot products are exact negatives (IEEE
+// rounding is sign-symmetric), so one evaluation gives both.
+//   Thread (tx, ty) owns tile cell (tx, ty), whose source fields (phi, u, n)
+// are kept in a 3-plane register window (z, z+1, z+2 in flight). On plane z
+// it writes its in-plane links and its c_z = -1 links from plane z+1, and
+// computes its c_z = +1 links from plane z, held (5 registers) for plane
+// z+1. Threads tid < gm_ring_cells also own one ring cell, of which they
+// compute only the five links that enter the tile: its plane z+1 is read at
+// the start of plane z and used after the tile cell's work; what later
+// planes need of it is held (4 values). This keeps a lane within 80 VGPRs,
+// i.e. three 8-wave workgroups per CU. Slots are compile-time (the z loop
+// is unrolled by 6, the period of the window and of the ring's held
+// values). Must be called by all TX * TY threads (it contains barriers).
 template <int TX, int TY>
 __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0, int zb, int ze) {
     constexpr int NT = TX * TY;
     constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
-    constexpr int KH = (HN + NT - 1) / NT;
+    constexpr int NR = gm_ring_cells<TX, TY>();
+    static_assert(NR <= NT, "a lane per ring cell");
     __shared__ float s_g[HN * Q];   // [halo cell][link]: g*_l of the source cell
 
     const Geo& G = c.geo;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    unsigned hoff[KH];
-#pragma unroll
-    for (int q = 0; q < KH; ++q) {
-        const int e = tid + q * NT;
-        unsigned off = 0;
-        if (e < HN) {
-            const int hx = e % HX, hy = e / HX;
-            off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
-        }
-        hoff[q] = off;
-    }
-    // win[slot][q]: phi, ux, uy, uz, nx, ny, nz of halo cell q on one plane.
-    float win[4][KH][7];
-    auto fetch = [&](float (&r)[KH][7], int z) {
-        const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
-#pragma unroll
-        for (int q = 0; q < KH; ++q) {
-            const unsigned o = base + hoff[q];
-            r[q][0] = ld32(c.phi_prev, o);
-            r[q][1] = ld32(c.gm_u[0], o);
-            r[q][2] = ld32(c.gm_u[1], o);
-            r[q][3] = ld32(c.gm_u[2], o);
-            r[q][4] = ld32(c.gm_n[0], o);
-            r[q][5] = ld32(c.gm_n[1], o);
-            r[q][6] = ld32(c.gm_n[2], o);
-        }
+    const float w1 = c.phys.w[1], w2 = c.phys.w[2];
+    auto fetch = [&](float (&r)[7], unsigned o) {
+        r[0] = ld32(c.phi_prev, o);
+        r[1] = ld32(c.gm_u[0], o);
+        r[2] = ld32(c.gm_u[1], o);
+        r[3] = ld32(c.gm_u[2], o);
+        r[4] = ld32(c.gm_n[0], o);
+        r[5] = ld32(c.gm_n[1], o);
+        r[6] = ld32(c.gm_n[2], o);
     };
+    auto plane_of = [&](int z) { return (unsigned)(wrapmod(z, G.nz) * G.nxy); };
+    auto cell = [&](const float (&r)[7]) { return phase_cell(c.phys, r[0], r[1], r[2], r[3], r[4], r[5], r[6]); };
+
+    // ---- tile cell
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
     const int e0 = (ty + 1) * HX + (tx + 1);
-    // Destination plane z with the window slots of planes z-1, z, z+1 (PM,
-    // P0, PP) and the slot that receives plane z+2 (PN).
-    auto iter = [&](int z, auto pm, auto p0, auto pp, auto pn) {
-        constexpr int PM = decltype(pm)::value, P0 = decltype(p0)::value, PP = decltype(pp)::value;
-        constexpr int PN = decltype(pn)::value;
-        if (z + 1 < ze) fetch(win[PN], z + 2);   // read by the next plane
+    // Cells past the lattice edge (partial tiles) take their periodic image.
+    const unsigned own = (unsigned)(wrapmod(y, G.ny) * G.nx + wrapmod(x, G.nx));
+    float f[3][7];     // window: phi, ux, uy, uz, nx, ny, nz on planes by slot
+    float hold[5];     // c_z = +1 links for plane z+1, from plane z
+    // ---- ring cell (side 0: row y0-1, 1: row y0+TY, 2: column x0-1, 3: column x0+TX)
+    const bool ring = tid < NR;
+    int hx, hy, side;
+    if (tid < TX) { hx = tid + 1; hy = 0; side = 0; }
+    else if (tid < 2 * TX) { hx = tid - TX + 1; hy = TY + 1; side = 1; }
+    else if (tid < 2 * TX + TY + 2) { hx = 0; hy = tid - 2 * TX; side = 2; }
+    else { hx = TX + 1; hy = min(tid - 2 * TX - TY - 2, TY + 1); side = 3; }
+    const unsigned roff = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
+    const int rbase = (hy * HX + hx) * Q;
+    // The side's five link numbers, 5 bits each.
+    unsigned rlinks = 0;
 #pragma unroll
-        for (int q = 0; q < KH; ++q) {
-            const int e = tid + q * NT;
-            if (q < KH - 1 || e < HN) {
-                auto cell = [&](const float (&f)[7]) {
-                    return phase_cell(c.phys, f[0], f[1], f[2], f[3], f[4], f[5], f[6]);
-                };
-                const PhaseCell cm = cell(win[PM][q]), c0 = cell(win[P0][q]), cp = cell(win[PP][q]);
-                float* dst = s_g + e * Q;
-                for_links([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    if constexpr (l == 0 || CZ[l] != 0) {
-                        dst[l] = phase_post<l>(CZ[l] > 0 ? cm : (CZ[l] < 0 ? cp : c0), c.phys);
-                    } else if constexpr (l % 2 == 1) {
-                        // In-plane pair (l, l+1), both from this plane's cell:
-                        // c_{l+1} = -c_l, so the dot products of link l+1 are
-                        // the exact negatives of link l's (IEEE rounding is
-                        // sign-symmetric) and phase_post<l+1> equals this.
-                        static_assert(CX[l + 1] == -CX[l] && CY[l + 1] == -CY[l] && CZ[l + 1] == 0, "pair");
-                        const float w = c.phys.w[WC[l]];
-                        const float ud = cdot<l>(c0.ux, c0.uy, c0.uz) * INV_CSSQ;
-                        const float wa = w * (c0.phi_norm * cdot<l>(c0.nx, c0.ny, c0.nz));
-                        const float wphi = w * c0.phi;
-                        dst[l] = wphi * (1.0f + ud) + wa;
-                        dst[l + 1] = wphi * (1.0f + -ud) + -wa;
-                    }
-                });
-            }
+    for (int k = 0; k < 5; ++k)
+        rlinks |= (unsigned)(side == 0 ? RING_LINK[0][k]
+                                       : side == 1 ? RING_LINK[1][k] : side == 2 ? RING_LINK[2][k] : RING_LINK[3][k])
+                  << (5 * k);
+    auto rslot = [&](int k) { return s_g + rbase + (int)((rlinks >> (5 * k)) & 31u); };
+    const bool ry = side < 2;                     // the side's axis is y
+    const float rs = (side & 1) ? -1.0f : 1.0f;   // and its sign
+    float rin[3];      // in-plane links of the ring cell for the next plane
+    float rhold[2];    // its c_z = +1 link for planes by parity
+    // From the ring cell's plane p: in-plane links (for plane p) into rin,
+    // c_z = -1 link (for plane p-1) returned, c_z = +1 link (plane p+1) to up.
+    auto ring_plane = [&](const float (&r)[7], float& up) {
+        const PhaseCell q = cell(r);
+        const float pu = rs * (ry ? q.uy : q.ux), qu = ry ? q.ux : q.uy;
+        const float pn = rs * (ry ? q.ny : q.nx), qn = ry ? q.nx : q.ny;
+        const float w1phi = w1 * q.phi, w2phi = w2 * q.phi;
+        rin[0] = phase_link(w1, w1phi, q.phi_norm, pu, pn);
+        rin[1] = phase_link(w2, w2phi, q.phi_norm, pu + qu, pn + qn);
+        rin[2] = phase_link(w2, w2phi, q.phi_norm, pu - qu, pn - qn);
+        up = phase_link(w2, w2phi, q.phi_norm, pu + q.uz, pn + q.nz);
+        return phase_link(w2, w2phi, q.phi_norm, pu - q.uz, pn - q.nz);
+    };
+
+    // Window slot of plane p: (p - zb) mod 3 (plane zb-1 passes through slot
+    // 2); the ring's held c_z = +1 link for plane p: rhold[(p - zb) mod 2].
+    fetch(f[2], plane_of(zb - 1) + own);
+    fetch(f[0], plane_of(zb) + own);
+    fetch(f[1], plane_of(zb + 1) + own);
+    {
+        const PhaseCell pm = cell(f[2]);
+        int k = 0;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (CZ[l] > 0) { hold[k] = phase_post<l>(pm, c.phys); ++k; }
+        });
+        if (ring) {
+            float r[7];
+            fetch(r, plane_of(zb - 1) + roff);
+            ring_plane(r, rhold[0]);
+            fetch(r, plane_of(zb) + roff);
+            ring_plane(r, rhold[1]);   // leaves plane zb's in-plane links in rin
+        }
+    }
+    auto iter = [&](int z, auto p0, auto pp, auto pn, auto ph) {
+        constexpr int P0 = decltype(p0)::value, PP = decltype(pp)::value, PN = decltype(pn)::value;
+        constexpr int PH = decltype(ph)::value;
+        float r[7];
+        if (ring) fetch(r, plane_of(z + 1) + roff);            // used after the tile cell's work
+        if (z + 1 < ze) fetch(f[PN], plane_of(z + 2) + own);   // read by the next plane
+        float* dst = s_g + e0 * Q;
+        {   // c_z = +1 links from plane z-1, computed on the previous plane
+            int k = 0;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] > 0) { dst[l] = hold[k]; ++k; }
+            });
+        }
+        {   // in-plane links from plane z
+            const PhaseCell c0 = cell(f[P0]);
+            dst[0] = phase_post<0>(c0, c.phys);
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (l != 0 && CZ[l] == 0 && l % 2 == 1) {
+                    static_assert(CX[l + 1] == -CX[l] && CY[l + 1] == -CY[l] && CZ[l + 1] == 0, "pair");
+                    const float wl = c.phys.w[WC[l]];
+                    const float ud = cdot<l>(c0.ux, c0.uy, c0.uz) * INV_CSSQ;
+                    const float wa = wl * (c0.phi_norm * cdot<l>(c0.nx, c0.ny, c0.nz));
+                    const float wphi = wl * c0.phi;
+                    dst[l] = wphi * (1.0f + ud) + wa;
+                    dst[l + 1] = wphi * (1.0f + -ud) + -wa;
+                }
+            });
+            int k = 0;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] > 0) { hold[k] = phase_post<l>(c0, c.phys); ++k; }
+            });
+        }
+        {   // c_z = -1 links from plane z+1
+            const PhaseCell cp = cell(f[PP]);
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] < 0) dst[l] = phase_post<l>(cp, c.phys);
+            });
+        }
+        if (ring) {
+            *rslot(3) = rhold[PH];
+            *rslot(0) = rin[0];
+            *rslot(1) = rin[1];
+            *rslot(2) = rin[2];
+            *rslot(4) = ring_plane(r, rhold[PH]);
         }
         __syncthreads();
         if (valid) {
@@ -304,23 +411,20 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
                 constexpr int l = decltype(lc)::value;
                 sum += s_g[(e0 - CY[l] * HX - CX[l]) * Q + l];
             });
-            st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), sum);
+            st32(c.phi, (unsigned)z * (unsigned)G.nxy + own, sum);
         }
         __syncthreads();   // s_g is rewritten by the next plane
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
-    using I3 = std::integral_constant<int, 3>;
-    // Slot of plane p: (p - (zb - 1)) mod 4.
-    fetch(win[0], zb - 1);
-    fetch(win[1], zb);
-    fetch(win[2], zb + 1);
-    for (int z = zb; z < ze; z += 4) {
-        iter(z, I0{}, I1{}, I2{}, I3{});
-        if (z + 1 < ze) iter(z + 1, I1{}, I2{}, I3{}, I0{});
-        if (z + 2 < ze) iter(z + 2, I2{}, I3{}, I0{}, I1{});
-        if (z + 3 < ze) iter(z + 3, I3{}, I0{}, I1{}, I2{});
+    for (int z = zb; z < ze; z += 6) {
+        iter(z, I0{}, I1{}, I2{}, I0{});
+        if (z + 1 < ze) iter(z + 1, I1{}, I2{}, I0{}, I1{});
+        if (z + 2 < ze) iter(z + 2, I2{}, I0{}, I1{}, I0{});
+        if (z + 3 < ze) iter(z + 3, I0{}, I1{}, I2{}, I1{});
+        if (z + 4 < ze) iter(z + 4, I1{}, I2{}, I0{}, I0{});
+        if (z + 5 < ze) iter(z + 5, I2{}, I0{}, I1{}, I1{});
     }
 }
 
