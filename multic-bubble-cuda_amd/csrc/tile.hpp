@@ -434,62 +434,116 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
 // the vector B (in gm_u) and the phase pass sums
 //   phi(x) = w0 phi(x) + w1 sum_faces (phi + c.B)(x - c) + w2 sum_edges (...).
 // 4 fields per source cell instead of 7, and a handful of adds per link.
-template <int TX, int TY>
-__device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0, int zb, int ze) {
+// The tile and its 1-cell halo are staged plane by plane into a 3-plane LDS
+// ring (the next plane's loads in flight while a plane is summed). PAIRED:
+// a lane stages two x-adjacent cells of an even-aligned pair (one 8-byte phi
+// load and one 4-byte load per B component, the halo in one pass); it needs
+// an even row length, so that every row, hence every pair, starts at an
+// even cell. Otherwise a lane stages single cells.
+template <int TX, int TY, bool PAIRED>
+__device__ __forceinline__ void phase_tile_ab_impl(const CaseView& c, int x0, int y0, int zb, int ze,
+                                                   float (*s_a)[4][(TX + 2) * (TY + 2)]) {
     constexpr int NT = TX * TY;
     constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
     constexpr int KH = (HN + NT - 1) / NT;
-    __shared__ float s_a[3][4][HN];   // phi, Bx, By, Bz
+    constexpr int PX = TX / 2 + 2, NPR = PX * HY;   // pairs per halo row, in the halo
+    static_assert(TX % 2 == 0 && NPR <= NT, "a lane per pair");
+    constexpr int KS = PAIRED ? 1 : KH;             // staging items per lane
 
     const Geo& G = c.geo;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const float w0 = c.phys.w[0], w1 = c.phys.w[1], w2 = c.phys.w[2];
-    unsigned hoff[KH];
+    // Staging items: single halo cells e = tid + q NT, or the pair starting
+    // at halo column 2 px - 1 of halo row hy (tid = hy PX + px).
+    unsigned hoff[KS];
+    int pe = 0;
 #pragma unroll
-    for (int q = 0; q < KH; ++q) {
-        const int e = tid + q * NT;
+    for (int q = 0; q < KS; ++q) {
         unsigned off = 0;
-        if (e < HN) {
-            const int hx = e % HX, hy = e / HX;
-            off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
+        if constexpr (PAIRED) {
+            if (tid < NPR) {
+                const int px = tid % PX, hy = tid / PX;
+                off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 2 + 2 * px, G.nx));
+                pe = hy * HX + 2 * px - 1;
+            }
+        } else {
+            const int e = tid + q * NT;
+            if (e < HN) {
+                const int hx = e % HX, hy = e / HX;
+                off = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx));
+            }
         }
         hoff[q] = off;
     }
-    // Software-pipelined staging, as in phase_tile_gm.
-    float reg[KH][4];
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef const f32x2 __attribute__((address_space(1)))* gf2;
+    typedef const unsigned __attribute__((address_space(1)))* gu32;
+    typedef const unsigned short __attribute__((address_space(1)))* gu16;
+    auto dec = [](unsigned short h) { return (float)__builtin_bit_cast(_Float16, h) * kBUnscale; };
+    float reg[KS][4];
+    unsigned rb[3];   // PAIRED: B components of the pair, two FP16 each
+    f32x2 rp;         // PAIRED: phi of the pair
     auto fetch = [&](int z) {
         const unsigned base = (unsigned)(wrapmod(z, G.nz) * G.nxy);
+        if constexpr (PAIRED) {
+            if (tid < NPR) {
+                const unsigned o = base + hoff[0];
+                rp = *((gf2)c.phi_prev + (o >> 1));
 #pragma unroll
-        for (int q = 0; q < KH; ++q) {
-            const int e = tid + q * NT;
-            if (q < KH - 1 || e < HN) {
-                const unsigned o = base + hoff[q];
-                reg[q][0] = ld32(c.phi_prev, o);
-                reg[q][1] = ld_b(c.gm_u[0], o);
-                reg[q][2] = ld_b(c.gm_u[1], o);
-                reg[q][3] = ld_b(c.gm_u[2], o);
+                for (int k = 0; k < 3; ++k) rb[k] = *(gu32)((gu16)c.gm_u[k] + o);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KS; ++q) {
+                const int e = tid + q * NT;
+                if (q < KS - 1 || e < HN) {
+                    const unsigned o = base + hoff[q];
+                    reg[q][0] = ld32(c.phi_prev, o);
+                    reg[q][1] = ld_b(c.gm_u[0], o);
+                    reg[q][2] = ld_b(c.gm_u[1], o);
+                    reg[q][3] = ld_b(c.gm_u[2], o);
+                }
             }
         }
     };
-    auto put = [&](int z) {
-        float (*dst)[HN] = s_a[wrapmod(z, 3)];
+    auto put = [&](int slot) {
+        float (*dst)[HN] = s_a[slot];
+        if constexpr (PAIRED) {
+            if (tid < NPR) {
+                // Halo columns 2 px - 1 (from px = 1) and 2 px (up to px = TX / 2).
+                const int px = tid % PX;
+                if (px >= 1) {
+                    dst[0][pe] = rp.x;
 #pragma unroll
-        for (int q = 0; q < KH; ++q) {
-            const int e = tid + q * NT;
-            if (q < KH - 1 || e < HN)
-                for (int k = 0; k < 4; ++k) dst[k][e] = reg[q][k];
+                    for (int k = 0; k < 3; ++k) dst[1 + k][pe] = dec((unsigned short)(rb[k] & 0xffffu));
+                }
+                if (px <= TX / 2) {
+                    dst[0][pe + 1] = rp.y;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dst[1 + k][pe + 1] = dec((unsigned short)(rb[k] >> 16));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KS; ++q) {
+                const int e = tid + q * NT;
+                if (q < KS - 1 || e < HN)
+                    for (int k = 0; k < 4; ++k) dst[k][e] = reg[q][k];
+            }
         }
     };
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
     const int e0 = (ty + 1) * HX + (tx + 1);
+    // Ring slots of planes z-1, z, z+1, rotated once per plane.
+    int sm = 0, s0 = 1, sp = 2;
     fetch(zb - 1);
-    put(zb - 1);
+    put(sm);
     fetch(zb);
-    put(zb);
+    put(s0);
     fetch(zb + 1);
     for (int z = zb; z < ze; ++z) {
-        put(z + 1);
+        put(sp);   // plane z+1, fetched during the previous plane
         __syncthreads();
         if (z + 1 < ze) fetch(z + 2);
         if (valid) {
@@ -497,7 +551,7 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
             for_links([&](auto lc) {
                 constexpr int l = decltype(lc)::value;
                 if constexpr (l != 0) {
-                    const float(*p)[HN] = s_a[wrapmod(z - CZ[l], 3)];
+                    const float(*p)[HN] = s_a[CZ[l] > 0 ? sm : (CZ[l] < 0 ? sp : s0)];
                     const int e = e0 - CY[l] * HX - CX[l];
                     float v = p[0][e];
                     if constexpr (CX[l] != 0) v = CX[l] > 0 ? v + p[1][e] : v - p[1][e];
@@ -507,11 +561,22 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
                     else edges += v;
                 }
             });
-            const float own = s_a[wrapmod(z, 3)][0][e0];
+            const float own = s_a[s0][0][e0];
             st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), w0 * own + w1 * faces + w2 * edges);
         }
-        __syncthreads();
+        __syncthreads();   // the slot of plane z-1 is rewritten next
+        const int t = sm;
+        sm = s0;
+        s0 = sp;
+        sp = t;
     }
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0, int zb, int ze) {
+    __shared__ float s_a[3][4][(TX + 2) * (TY + 2)];   // ring of 3 planes: phi, Bx, By, Bz
+    if ((c.geo.nx & 1) == 0) phase_tile_ab_impl<TX, TY, true>(c, x0, y0, zb, ze, s_a);
+    else phase_tile_ab_impl<TX, TY, false>(c, x0, y0, zb, ze, s_a);
 }
 
 // ---------------------------------------------------------------- collide
