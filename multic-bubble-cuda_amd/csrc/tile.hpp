@@ -245,7 +245,7 @@ __device__ __forceinline__ float phase_link(float w, float wphi, float phi_norm,
 }
 
 // Cells of the 1-cell ring around a TX x TY tile that send links into it
-// (the four corners only along the columns' sides).
+// (two rows of TX cells and two columns of TY + 2, corners included).
 template <int TX, int TY> constexpr int gm_ring_cells() { return 2 * TX + 2 * (TY + 2); }
 
 // Moment-form phase pass for the TX x TY column (x0, y0) over planes
@@ -265,7 +265,7 @@ template <int TX, int TY> constexpr int gm_ring_cells() { return 2 * TX + 2 * (T
 // z+1. Threads tid < gm_ring_cells also own one ring cell, of which they
 // compute only the five links that enter the tile: its plane z+1 is read at
 // the start of plane z and used after the tile cell's work; what later
-// planes need of it is held (4 values). This keeps a lane within 80 VGPRs,
+// planes need of it is held (5 values). This keeps a lane within 80 VGPRs,
 // i.e. three 8-wave workgroups per CU. Slots are compile-time (the z loop
 // is unrolled by 6, the period of the window and of the ring's held
 // values). Must be called by all TX * TY threads (it contains barriers).
