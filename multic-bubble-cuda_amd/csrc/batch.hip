@@ -99,11 +99,19 @@ Tiling choose_tiling(const mcb_params& p) {
     Tiling t = p.nx >= 64 ? Tiling{64, 8, 16} : (p.nx >= 32 ? Tiling{32, 8, 16} : Tiling{16, 16, 16});
     // 16-bit forms: half the bytes per cell, so longer z runs pay off
     // (measured at 256^3: 64x8x32 fastest, 64x4x32 4% and 32x8x32 17%
-    // slower); a 32-wide tile where it pads x less (e.g. nx = 80, 144, 160),
-    // since padded lanes do no work.
+    // slower); a 32-wide tile where it pads x less, since padded lanes do no
+    // work. Measured per x side on batches of equal cubes (48 ... 192): a tile
+    // spanning the whole row where one is instantiated (48, 80, 96, 112, 144,
+    // 176; 1.1-1.9x), 64x4 for the other multiples of 32 up to 192; both with
+    // z-chunks of 16, which give a lattice of this size alone enough
+    // workgroups to fill the GPU (serving at 20 requests/s: p99 0.13 s
+    // against 0.28 s with chunks of 32, and the same throughput at saturation).
     if (p.storage != MCB_STORE_FP32 && p.nx >= 48) {
         const int pad64 = (p.nx + 63) / 64 * 64, pad32 = (p.nx + 31) / 32 * 32;
         t = pad32 < pad64 ? Tiling{32, 8, 32} : Tiling{64, 8, 32};
+        const Tiling row{p.nx, 4, 16};
+        if (tiling_supported16(row)) t = row;
+        else if (p.nx <= 192 && p.nx % 32 == 0) t = Tiling{64, 4, 16};
     }
     t.zc = std::min(t.zc, p.nz);
     return t;

@@ -238,10 +238,9 @@ __global__ __launch_bounds__(256) void k_add_scalar(float* __restrict__ f, long 
 
 dim3 cell_grid(const Geo& G) { return dim3((G.nx + PBX - 1) / PBX, (G.ny + PBY - 1) / PBY, G.nz); }
 
-// Tile shapes the collide pass is instantiated for.
-const Tiling kTilings[] = {
-    {64, 8, 0}, {64, 4, 0}, {32, 8, 0}, {32, 4, 0}, {16, 16, 0}, {16, 8, 0},
-};
+#define MCB_TILE_ENTRY(TX, TY) {TX, TY, 0},
+const Tiling kTilings[] = {MCB_TILES_FP32(MCB_TILE_ENTRY)};
+#undef MCB_TILE_ENTRY
 
 template <int TX, int TY>
 hipError_t launch_collide_t(const CaseView& c, int mode, hipStream_t s) {
@@ -283,12 +282,10 @@ hipError_t launch_batch_t(const CaseView* cases, const WorkItem* work, unsigned 
 hipError_t launch_batch(const CaseView* cases, const WorkItem* work, unsigned n, const Tiling& t, int mode,
                         bool phase, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (t.tx == 64 && t.ty == 8) return launch_batch_t<64, 8>(cases, work, n, mode, phase, s);
-    if (t.tx == 64 && t.ty == 4) return launch_batch_t<64, 4>(cases, work, n, mode, phase, s);
-    if (t.tx == 32 && t.ty == 8) return launch_batch_t<32, 8>(cases, work, n, mode, phase, s);
-    if (t.tx == 32 && t.ty == 4) return launch_batch_t<32, 4>(cases, work, n, mode, phase, s);
-    if (t.tx == 16 && t.ty == 16) return launch_batch_t<16, 16>(cases, work, n, mode, phase, s);
-    if (t.tx == 16 && t.ty == 8) return launch_batch_t<16, 8>(cases, work, n, mode, phase, s);
+#define MCB_TILE_CASE(TX, TY) \
+    if (t.tx == TX && t.ty == TY) return launch_batch_t<TX, TY>(cases, work, n, mode, phase, s);
+    MCB_TILES_FP32(MCB_TILE_CASE)
+#undef MCB_TILE_CASE
     return hipErrorInvalidValue;
 }
 
@@ -314,12 +311,7 @@ hipError_t launch_phase(const CaseView& c0, const Tiling& t, hipStream_t s) {
         hipLaunchKernelGGL((k_phase_gm<TX, TY>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);            \
         return hipGetLastError();                                                                   \
     }
-        MCB_PHASE_GM(64, 8)
-        MCB_PHASE_GM(64, 4)
-        MCB_PHASE_GM(32, 8)
-        MCB_PHASE_GM(32, 4)
-        MCB_PHASE_GM(16, 16)
-        MCB_PHASE_GM(16, 8)
+        MCB_TILES_FP32(MCB_PHASE_GM)
 #undef MCB_PHASE_GM
         return hipErrorInvalidValue;
     }
@@ -332,12 +324,10 @@ hipError_t launch_phase(const CaseView& c0, const Tiling& t, hipStream_t s) {
 hipError_t launch_collide(const CaseView& c0, int mode, const Tiling& t, hipStream_t s) {
     CaseView c = c0;
     c.zc = t.zc;
-    if (t.tx == 64 && t.ty == 8) return launch_collide_t<64, 8>(c, mode, s);
-    if (t.tx == 64 && t.ty == 4) return launch_collide_t<64, 4>(c, mode, s);
-    if (t.tx == 32 && t.ty == 8) return launch_collide_t<32, 8>(c, mode, s);
-    if (t.tx == 32 && t.ty == 4) return launch_collide_t<32, 4>(c, mode, s);
-    if (t.tx == 16 && t.ty == 16) return launch_collide_t<16, 16>(c, mode, s);
-    if (t.tx == 16 && t.ty == 8) return launch_collide_t<16, 8>(c, mode, s);
+#define MCB_TILE_CASE(TX, TY) \
+    if (t.tx == TX && t.ty == TY) return launch_collide_t<TX, TY>(c, mode, s);
+    MCB_TILES_FP32(MCB_TILE_CASE)
+#undef MCB_TILE_CASE
     return hipErrorInvalidValue;
 }
 

@@ -85,6 +85,16 @@ struct WorkItem {
     int c, x0, y0, z0;
 };
 
+// Tile shapes (TX, TY) the kernels are instantiated for, as X(TX, TY) lists.
+// The FP32 list is also the autotuner's candidate list. The TX x 4 shapes
+// with TX = 48 ... 176 span a whole row of lattices with that x side: one
+// contiguous run per tile row and no padded lanes, which measured 1.1-1.9x
+// the 64- and 32-wide tiles for the 16-bit forms on such sides.
+#define MCB_TILES_FP32(X) \
+    X(64, 8) X(64, 4) X(32, 8) X(32, 4) X(16, 16) X(16, 8) X(48, 4) X(80, 4) X(96, 4) X(112, 4) X(144, 4) X(176, 4)
+#define MCB_TILES_16(X) \
+    X(64, 8) X(64, 4) X(32, 8) X(16, 16) X(48, 4) X(80, 4) X(96, 4) X(112, 4) X(144, 4) X(176, 4)
+
 bool tiling_supported(const Tiling& t);
 int tiling_count();
 Tiling tiling_at(int i);

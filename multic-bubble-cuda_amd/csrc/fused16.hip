@@ -148,10 +148,10 @@ template <class S>
 hipError_t batch_s(const CaseView* cases, const WorkItem* work, unsigned n, const Tiling& t, int mode, bool phase,
                    hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (t.tx == 64 && t.ty == 8) return batch_t<64, 8, S>(cases, work, n, mode, phase, s);
-    if (t.tx == 64 && t.ty == 4) return batch_t<64, 4, S>(cases, work, n, mode, phase, s);
-    if (t.tx == 32 && t.ty == 8) return batch_t<32, 8, S>(cases, work, n, mode, phase, s);
-    if (t.tx == 16 && t.ty == 16) return batch_t<16, 16, S>(cases, work, n, mode, phase, s);
+#define MCB_TILE_CASE(TX, TY) \
+    if (t.tx == TX && t.ty == TY) return batch_t<TX, TY, S>(cases, work, n, mode, phase, s);
+    MCB_TILES_16(MCB_TILE_CASE)
+#undef MCB_TILE_CASE
     return hipErrorInvalidValue;
 }
 
@@ -165,8 +165,11 @@ hipError_t batch16(const CaseView* cases, const WorkItem* work, unsigned n, cons
 }  // namespace
 
 bool tiling_supported16(const Tiling& t) {
-    return t.zc >= 1 && ((t.tx == 64 && (t.ty == 8 || t.ty == 4)) || (t.tx == 32 && t.ty == 8) ||
-                         (t.tx == 16 && t.ty == 16));
+#define MCB_TILE_CASE(TX, TY) \
+    if (t.tx == TX && t.ty == TY) return t.zc >= 1;
+    MCB_TILES_16(MCB_TILE_CASE)
+#undef MCB_TILE_CASE
+    return false;
 }
 
 hipError_t launch_phase_batch16(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
