@@ -9,8 +9,9 @@
 //                 g*_l(y) = w_l phi(y) + t_l(y) / K_l       (tau_g != 1)
 //               where phi(y) is the phase the source cell had when it
 //               collided (an FP32 field). At tau_g = 1 g is not stored as
-//               populations at all but in the FP32 moment form (phi, u, n;
-//               see phase_tile_gm). The offsets are small (density, velocity
+//               populations at all but in a moment form: (phi, u, n) in FP32
+//               for StoreF32 (phase_tile_gm), (phi, B) with B in FP16 for the
+//               16-bit forms (phase_tile_ab). The offsets are small (density, velocity
 //               and sharpening terms), so 16 bits keep about 1e-5 of w_l
 //               (FP16). K_l is a power of two near 2^12 / w_l: an
 //               exact scale, so encoding and decoding carry no systematic
