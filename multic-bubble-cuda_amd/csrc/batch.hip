@@ -1,6 +1,7 @@
 // Batched execution (batch.hpp) and the tile autotuner.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <tuple>
 
@@ -54,6 +55,8 @@ CaseView CaseState::view(int q) const {
     v.geo = geo;
     v.phys = phys;
     v.zc = tiling.zc;
+    v.tx = tiling.tx;
+    v.ty = tiling.ty;
     v.fsrc = f[q];
     v.fdst = f[q ^ 1];
     v.gsrc = g[q];
@@ -62,6 +65,7 @@ CaseView CaseState::view(int q) const {
     v.phi_prev = phi[q ^ 1];
     v.mslot = q;
     v.mass = mass;
+    v.massq = massq;
     v.partial = partial;
     // Moment form: only the parity of the initial state (cur = 0 before the
     // first step) sees GM_INITIAL.
@@ -283,6 +287,13 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
             if ((st = alloc((size_t)P * (fp32 ? sizeof(float) : 2), &q)) != MCB_OK) return st;
             (k < 3 ? c->gmu[k] : c->gmn[k - 3]) = static_cast<float*>(q);
         }
+        if (!fp32) {
+            // Mass-drift correction of the 16-bit moment form (MassQ).
+            if ((st = alloc(sizeof(MassQ), &q)) != MCB_OK) return st;
+            c->massq = static_cast<MassQ*>(q);
+            const MassQ zero{0, 0, 0, {0.0, 0.0}};
+            B_HIP(hipMemcpyAsync(c->massq, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_));
+        }
         c->gm = GM_INITIAL;
     } else if (p.storage != MCB_STORE_FP32) {
         // 16-bit phase populations: per-step FP64 mass-drift correction.
@@ -336,7 +347,7 @@ mcb_status Batch::build_own_table(CaseState* c) {
         c->work_cap = own.size();
     }
     B_HIP(hipMemcpyAsync(c->d_work, own.data(), own.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream_));
-    if (c->mass && c->n_work > c->partial_cap) {
+    if ((c->mass || c->massq) && c->n_work > c->partial_cap) {
         if (c->partial) B_HIP(hipFree(c->partial));
         c->partial = nullptr;
         c->partial_cap = 0;
@@ -408,6 +419,7 @@ mcb_status Batch::rebuild() {
     }
     std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_group;
     any_initial_ = false;
+    any_massq_ = false;
     for (int q = 0; q < 2; ++q) hv_[q].clear();
     for (size_t i = 0; i < n; ++i) {
         CaseState* c = cases_[order_[i]].get();
@@ -418,6 +430,7 @@ mcb_status Batch::rebuild() {
             hv_[q].push_back(v);
         }
         if (c->gm == GM_INITIAL) any_initial_ = true;
+        if (c->massq) any_massq_ = true;
         auto tl = tiles_of(*c, (int)i);
         auto& vec = by_group[std::make_tuple(c->tiling.tx, c->tiling.ty, c->mode)];
         vec.insert(vec.end(), tl.begin(), tl.end());
@@ -484,6 +497,7 @@ mcb_status Batch::launch_step(int q) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_phase_batch(d_views_[q], gr.d_work, gr.n, gr.t, storage_, stream_));
     }
+    if (any_massq_) B_HIP(launch_massq_finish(d_views_[q], (unsigned)order_.size(), stream_));
     for (auto& kv : groups_) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_collide_batch(d_views_[q], gr.d_work, gr.n, gr.t, gr.mode, storage_, stream_));
@@ -628,7 +642,7 @@ mcb_status Batch::to_populations(CaseState* c) {
 // g as populations, or its moment form (u, n in FP32; B in 16 bits); the
 // state's phi (the moment form's phi, the 16-bit g anchor); the 16-bit mass
 // target and the correction pending for the next phase pass.
-std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass2) {
+std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     const unsigned eb = pop_bytes(c->storage);
     std::vector<StatePart> parts;
@@ -643,15 +657,16 @@ std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass2) {
         for (int k = 0; k < 3; ++k)
             if (c->gmn[k]) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
     }
-    if (c->mass) parts.push_back({"mass", mass2, 1, 2, 2, 8, true});
+    if (c->mass) parts.push_back({"mass", mass, 1, 2, 2, 8, true});
+    if (c->massq) parts.push_back({"massq", mass + 2, 1, 3, 3, 8, true});
     return parts;
 }
 
 mcb_status Batch::state_size(int id, size_t* bytes) {
     CaseState* c = get(id);
     if (!c || !bytes) return fail(MCB_ERR_INVALID_ARG, "no such case or null size pointer");
-    double m2[2];
-    *bytes = state_image_bytes(state_parts(c, m2));
+    double m5[5];
+    *bytes = state_image_bytes(state_parts(c, m5));
     return MCB_OK;
 }
 
@@ -660,19 +675,26 @@ mcb_status Batch::save_state(int id, void* buf, size_t bytes) {
     if (!c) return fail(MCB_ERR_INVALID_ARG, "no such case");
     B_HIP(hipSetDevice(device_));
     B_HIP(hipStreamSynchronize(stream_));
-    double m2[2] = {0.0, 0.0};
+    double m5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (c->mass) {
         MassCell m;
         B_HIP(hipMemcpy(&m, c->mass, sizeof(m), hipMemcpyDeviceToHost));
-        m2[0] = m.target;
-        m2[1] = m.delta[c->cur];
+        m5[0] = m.target;
+        m5[1] = m.delta[c->cur];
+    }
+    if (c->massq) {
+        MassQ m;
+        B_HIP(hipMemcpy(&m, c->massq, sizeof(m), hipMemcpyDeviceToHost));
+        std::memcpy(&m5[2], &m.target, sizeof(double));   // the integer's bits
+        m5[3] = (double)m.set;
+        m5[4] = m.delta[c->cur];
     }
     StateHeader h{};
     h.params = c->p;
     h.path = MCB_PATH_FUSED;
     h.gm = c->gm;
     h.steps = c->steps;
-    return state_image_save(stream_, h, state_parts(c, m2), buf, bytes, &err_);
+    return state_image_save(stream_, h, state_parts(c, m5), buf, bytes, &err_);
 }
 
 mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
@@ -691,12 +713,18 @@ mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
     // A case that holds populations and gets a moment-form image keeps its
     // g buffers (unused) and reads the moment fields again.
     c->gm = h.gm;
-    double m2[2] = {0.0, 0.0};
-    if ((st = state_image_load(stream_, buf, bytes, state_parts(c, m2), &err_)) != MCB_OK) return st;
+    double m5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if ((st = state_image_load(stream_, buf, bytes, state_parts(c, m5), &err_)) != MCB_OK) return st;
     if (c->mass) {
-        MassCell m{m2[0], {0.0, 0.0}, 0u, 0u};
-        m.delta[c->cur] = m2[1];
+        MassCell m{m5[0], {0.0, 0.0}, 0u, 0u};
+        m.delta[c->cur] = m5[1];
         B_HIP(hipMemcpy(c->mass, &m, sizeof(m), hipMemcpyHostToDevice));
+    }
+    if (c->massq) {
+        MassQ m{0, m5[3] != 0.0 ? 1 : 0, 0, {0.0, 0.0}};
+        std::memcpy(&m.target, &m5[2], sizeof(double));
+        m.delta[c->cur] = m5[4];
+        B_HIP(hipMemcpy(c->massq, &m, sizeof(m), hipMemcpyHostToDevice));
     }
     c->steps = h.steps;
     c->phi_last = c->phi[c->cur ^ 1];

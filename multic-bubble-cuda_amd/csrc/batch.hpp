@@ -39,6 +39,7 @@ struct CaseState {
     float* gmu[3] = {nullptr, nullptr, nullptr};
     float* gmn[3] = {nullptr, nullptr, nullptr};
     MassCell* mass = nullptr;       // 16-bit storage: mass-drift correction state
+    MassQ* massq = nullptr;         // 16-bit storage in moment form: its correction state
     double* partial = nullptr;      // one FP64 partial per phase work item
     size_t partial_cap = 0;
     std::vector<std::pair<void*, size_t>> allocs;   // device blocks and their sizes
@@ -117,9 +118,11 @@ private:
     // Keep g as populations from here on (arbitrary g has no moment form):
     // allocates the g buffers and, for 16 bits, the mass-drift state.
     mcb_status to_populations(CaseState* c);
-    // The arrays of a case's state image; mass2 receives / supplies the
-    // 16-bit mass target and pending correction.
-    std::vector<StatePart> state_parts(CaseState* c, double* mass2);
+    // The arrays of a case's state image; mass receives / supplies (5 slots)
+    // the 16-bit mass target and pending correction of stored g (0, 1) and
+    // of the moment form (2: fixed-point target, 3: target recorded, 4:
+    // pending correction).
+    std::vector<StatePart> state_parts(CaseState* c, double* mass);
     // One set of macro fields per batch, sized for its largest case; it holds
     // the observables of the case of the most recent macros() call.
     mcb_status ensure_macro_scratch();
@@ -150,6 +153,7 @@ private:
     std::map<std::tuple<int, int, int>, Group> groups_;
     std::vector<CaseView> hv_[2];     // host copies of the view arrays
     bool any_initial_ = false;        // a moment-form case has not stepped yet
+    bool any_massq_ = false;          // a case keeps a moment-form mass correction (MassQ)
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     double* d_partial_ = nullptr;     // reduction scratch (2 * kReduceBlocks + 2 doubles)
     std::vector<void*> macro_bufs_;

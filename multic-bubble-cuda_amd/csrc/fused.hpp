@@ -44,13 +44,38 @@ struct MassCell {
     unsigned reserved;
 };
 
+// Mass-drift correction of the 16-bit storage forms in moment form (tau_g =
+// 1): the phase pass sums phi as 64-bit fixed-point integers (kMassQScale
+// per unit), an exact and hence order-independent sum, so the total and the
+// correction do not depend on how cells fall into workgroups (the step stays
+// the same bits under shifts and tilings). Each workgroup stores its partial;
+// a small kernel after the pass (launch_massq_finish) sums them and records
+// the first moment-form pass's total as the target, or sets the correction
+// delta that the next pass adds to every phi. (No workgroup waits on others:
+// an agent-scope release per workgroup would write back its XCD's L2.)
+struct MassQ {
+    long long target;   // fixed-point sum of phi to keep
+    int set;            // target recorded
+    int reserved;
+    double delta[2];    // per-cell correction the next phase pass adds, by state parity
+};
+// 2^31 per unit. A cell whose phi is below half the resolution counts as
+// zero, so what the correction adds there is invisible to the sum, and the
+// loop keeps adding until such cells reach half the resolution: at most
+// 2^-32 per cell (4e-3 on 256^3, ~3e-8 of the benchmark's total phase); at
+// 2^-20 this was 8 and ran away. The sum stays below 2^63 up to 3.6e9 cells
+// at phi <= 1.2.
+constexpr double kMassQScale = 2147483648.0;
+
 // Everything a kernel needs about one case (one lattice).
 struct CaseView {
     Geo geo;
     Phys phys;
     int zc;                 // z planes per collide workgroup
+    int tx, ty;             // tile of the case's work items
     int mslot;              // parity of the case's state: delta slot the phase pass reads
     MassCell* mass;         // 16-bit storage only
+    MassQ* massq;           // 16-bit storage in moment form only
     double* partial;        // one per work item of the case's tiling
     const void* fsrc;       // populations of the previous step (pull form)
     void* fdst;             // populations of this step
@@ -116,6 +141,9 @@ hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int s
 
 // ---- batched (many cases, one launch per pass), any storage form
 // storage: MCB_STORE_FP32 / FP16 / BF16; work: device array of n_work items.
+// After a batch's phase pass: the mass-drift corrections of its 16-bit
+// moment-form cases (MassQ), one workgroup per case of the view array.
+hipError_t launch_massq_finish(const CaseView* cases, unsigned n_cases, hipStream_t s);
 hipError_t launch_phase_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
                               int storage, hipStream_t s);
 hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,

@@ -172,6 +172,38 @@ bool tiling_supported16(const Tiling& t) {
     return false;
 }
 
+namespace {
+// launch_massq_finish: one workgroup per case of the view array.
+constexpr int kFinishThreads = 256;
+__global__ __launch_bounds__(kFinishThreads) void k_massq_finish(const CaseView* __restrict__ cases) {
+    const CaseView c = cases[blockIdx.x];
+    if (!c.massq || c.gm != GM_MOMENTS) return;   // uniform per workgroup
+    const Geo& G = c.geo;
+    const unsigned n_tiles = (unsigned)((G.nx + c.tx - 1) / c.tx) * (unsigned)((G.ny + c.ty - 1) / c.ty) *
+                             (unsigned)((G.nz + c.zc - 1) / c.zc);
+    const long long* partial = reinterpret_cast<const long long*>(c.partial);
+    long long acc = 0;
+    for (unsigned i = threadIdx.x; i < n_tiles; i += kFinishThreads) acc += partial[i];
+    const long long total = block_sum_q<kFinishThreads>(acc);
+    if (threadIdx.x == 0) {
+        MassQ* m = c.massq;
+        if (!m->set) {
+            m->target = total;
+            m->set = 1;
+            m->delta[c.mslot ^ 1] = 0.0;
+        } else {
+            m->delta[c.mslot ^ 1] = (double)(m->target - total) / kMassQScale / (double)G.n;
+        }
+    }
+}
+}  // namespace
+
+hipError_t launch_massq_finish(const CaseView* cases, unsigned n_cases, hipStream_t s) {
+    if (n_cases == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_massq_finish, dim3(n_cases), dim3(kFinishThreads), 0, s, cases);
+    return hipGetLastError();
+}
+
 hipError_t launch_phase_batch16(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
                                 int storage, hipStream_t s) {
     return batch16(cases, work, n_work, t, 0, true, storage, s);

@@ -429,6 +429,33 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
     }
 }
 
+// Sum of a 64-bit integer over the workgroup (exact, hence the same in any
+// order); valid in thread 0. Contains a barrier.
+template <int NT> __device__ __forceinline__ long long block_sum_q(long long v) {
+    static_assert(NT % 64 == 0, "whole waves");
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __shared__ long long s_wq[NT / 64];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if (tid % 64 == 0) s_wq[tid / 64] = v;
+    __syncthreads();
+    long long s = 0;
+    if (tid == 0)
+        for (int i = 0; i < NT / 64; ++i) s += s_wq[i];
+    return s;
+}
+
+// A moment-form phase work item's fixed-point partial sum of phi (see
+// MassQ), stored for launch_massq_finish.
+template <int TX, int TY>
+__device__ __forceinline__ void massq_partial(const CaseView& c, int x0, int y0, int z0, long long part) {
+    const Geo& G = c.geo;
+    const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
+    const unsigned tile = ((unsigned)(z0 / c.zc) * gy + (unsigned)(y0 / TY)) * gx + (unsigned)(x0 / TX);
+    const long long s = block_sum_q<TX * TY>(part);
+    if (threadIdx.y == 0 && threadIdx.x == 0) reinterpret_cast<long long*>(c.partial)[tile] = s;
+}
+
 // Moment form of the 16-bit storage forms (not bit-exact, so the algebra is
 // free): collisionPhase's g*_l(y) = w_l phi (1 + 3 c_l.u) + w_l phi_n c_l.n is
 // w_l (phi + c_l.B) with B = 3 phi u + phi_n n, so the state keeps phi and
@@ -440,7 +467,9 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
 // a lane stages two x-adjacent cells of an even-aligned pair (one 8-byte phi
 // load and one 4-byte load per B component, the halo in one pass); it needs
 // an even row length, so that every row, hence every pair, starts at an
-// even cell. Otherwise a lane stages single cells.
+// even cell. Otherwise a lane stages single cells. With a MassQ (16-bit
+// storage) every phi gets the pending mass-drift correction and enters the
+// fixed-point sum of the pass.
 template <int TX, int TY, bool PAIRED>
 __device__ __forceinline__ void phase_tile_ab_impl(const CaseView& c, int x0, int y0, int zb, int ze,
                                                    float (*s_a)[4][(TX + 2) * (TY + 2)]) {
@@ -536,6 +565,9 @@ __device__ __forceinline__ void phase_tile_ab_impl(const CaseView& c, int x0, in
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < G.nx && y < G.ny;
     const int e0 = (ty + 1) * HX + (tx + 1);
+    const bool mq = c.massq != nullptr;   // uniform per case
+    const float delta = mq ? (float)c.massq->delta[c.mslot] : 0.0f;
+    long long qpart = 0;
     // Ring slots of planes z-1, z, z+1, rotated once per plane.
     int sm = 0, s0 = 1, sp = 2;
     fetch(zb - 1);
@@ -563,7 +595,12 @@ __device__ __forceinline__ void phase_tile_ab_impl(const CaseView& c, int x0, in
                 }
             });
             const float own = s_a[s0][0][e0];
-            st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), w0 * own + w1 * faces + w2 * edges);
+            float v = w0 * own + w1 * faces + w2 * edges;
+            if (mq) {
+                v += delta;
+                qpart += (long long)__builtin_llrint((double)v * kMassQScale);
+            }
+            st32(c.phi, (unsigned)(z * G.nxy + y * G.nx + x), v);
         }
         __syncthreads();   // the slot of plane z-1 is rewritten next
         const int t = sm;
@@ -571,6 +608,7 @@ __device__ __forceinline__ void phase_tile_ab_impl(const CaseView& c, int x0, in
         s0 = sp;
         sp = t;
     }
+    if (mq) massq_partial<TX, TY>(c, x0, y0, zb, qpart);
 }
 
 template <int TX, int TY>
