@@ -374,7 +374,7 @@ mcb_status Batch::remove(int id) {
 }
 
 mcb_status Batch::set_mass_target(CaseState* c, double target) {
-    const MassCell m{target, {0.0, 0.0}, 0u, 0u};
+    const MassCell m{target, {0.0, 0.0}};
     B_HIP(hipMemcpyAsync(c->mass, &m, sizeof(m), hipMemcpyHostToDevice, stream_));
     B_HIP(hipStreamSynchronize(stream_));
     return MCB_OK;
@@ -419,7 +419,7 @@ mcb_status Batch::rebuild() {
     }
     std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_group;
     any_initial_ = false;
-    any_massq_ = false;
+    any_mass_ = false;
     for (int q = 0; q < 2; ++q) hv_[q].clear();
     for (size_t i = 0; i < n; ++i) {
         CaseState* c = cases_[order_[i]].get();
@@ -430,7 +430,7 @@ mcb_status Batch::rebuild() {
             hv_[q].push_back(v);
         }
         if (c->gm == GM_INITIAL) any_initial_ = true;
-        if (c->massq) any_massq_ = true;
+        if (c->massq || c->mass) any_mass_ = true;
         auto tl = tiles_of(*c, (int)i);
         auto& vec = by_group[std::make_tuple(c->tiling.tx, c->tiling.ty, c->mode)];
         vec.insert(vec.end(), tl.begin(), tl.end());
@@ -497,7 +497,7 @@ mcb_status Batch::launch_step(int q) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_phase_batch(d_views_[q], gr.d_work, gr.n, gr.t, storage_, stream_));
     }
-    if (any_massq_) B_HIP(launch_massq_finish(d_views_[q], (unsigned)order_.size(), stream_));
+    if (any_mass_) B_HIP(launch_massq_finish(d_views_[q], (unsigned)order_.size(), stream_));
     for (auto& kv : groups_) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_collide_batch(d_views_[q], gr.d_work, gr.n, gr.t, gr.mode, storage_, stream_));
@@ -716,7 +716,7 @@ mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
     double m5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if ((st = state_image_load(stream_, buf, bytes, state_parts(c, m5), &err_)) != MCB_OK) return st;
     if (c->mass) {
-        MassCell m{m5[0], {0.0, 0.0}, 0u, 0u};
+        MassCell m{m5[0], {0.0, 0.0}};
         m.delta[c->cur] = m5[1];
         B_HIP(hipMemcpy(c->mass, &m, sizeof(m), hipMemcpyHostToDevice));
     }

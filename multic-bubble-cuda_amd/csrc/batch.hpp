@@ -153,7 +153,7 @@ private:
     std::map<std::tuple<int, int, int>, Group> groups_;
     std::vector<CaseView> hv_[2];     // host copies of the view arrays
     bool any_initial_ = false;        // a moment-form case has not stepped yet
-    bool any_massq_ = false;          // a case keeps a moment-form mass correction (MassQ)
+    bool any_mass_ = false;          // a case keeps a mass-drift correction (MassCell or MassQ)
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     double* d_partial_ = nullptr;     // reduction scratch (2 * kReduceBlocks + 2 doubles)
     std::vector<void*> macro_bufs_;

@@ -32,16 +32,13 @@ struct Tiling {
 };
 
 // Mass bookkeeping of a case with 16-bit populations (device memory). The
-// phase pass adds delta[q] to every phi it computes at case parity q, writes
-// one FP64 partial sum of phi per work item, and the last work item to
-// finish sums the partials in a fixed order and stores
-// delta[q ^ 1] = (target - sum phi) / cells for the next step. The counter
-// returns to 0 after every pass.
+// phase pass adds delta[q] to every phi it computes at case parity q and
+// writes one FP64 partial sum of phi per work item; after the pass
+// launch_massq_finish sums the partials in a fixed order and stores
+// delta[q ^ 1] = (target - sum phi) / cells for the next step.
 struct MassCell {
     double target;          // sum of phi the case must keep
     double delta[2];
-    unsigned counter;
-    unsigned reserved;
 };
 
 // Mass-drift correction of the 16-bit storage forms in moment form (tau_g =
@@ -142,7 +139,8 @@ hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int s
 // ---- batched (many cases, one launch per pass), any storage form
 // storage: MCB_STORE_FP32 / FP16 / BF16; work: device array of n_work items.
 // After a batch's phase pass: the mass-drift corrections of its 16-bit
-// moment-form cases (MassQ), one workgroup per case of the view array.
+// cases (MassCell for stored g, MassQ for the moment form), one workgroup
+// per case of the view array.
 hipError_t launch_massq_finish(const CaseView* cases, unsigned n_cases, hipStream_t s);
 hipError_t launch_phase_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
                               int storage, hipStream_t s);

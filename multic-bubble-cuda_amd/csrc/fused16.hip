@@ -177,10 +177,17 @@ namespace {
 constexpr int kFinishThreads = 256;
 __global__ __launch_bounds__(kFinishThreads) void k_massq_finish(const CaseView* __restrict__ cases) {
     const CaseView c = cases[blockIdx.x];
-    if (!c.massq || c.gm != GM_MOMENTS) return;   // uniform per workgroup
     const Geo& G = c.geo;
     const unsigned n_tiles = (unsigned)((G.nx + c.tx - 1) / c.tx) * (unsigned)((G.ny + c.ty - 1) / c.ty) *
                              (unsigned)((G.nz + c.zc - 1) / c.zc);
+    if (c.mass && c.gm == GM_POPULATIONS) {   // stored 16-bit g: FP64 partials, fixed order
+        double acc = 0.0;
+        for (unsigned i = threadIdx.x; i < n_tiles; i += kFinishThreads) acc += c.partial[i];
+        const double total = block_sum<kFinishThreads>(acc);
+        if (threadIdx.x == 0) c.mass->delta[c.mslot ^ 1] = (c.mass->target - total) / (double)G.n;
+        return;
+    }
+    if (!c.massq || c.gm != GM_MOMENTS) return;   // uniform per workgroup
     const long long* partial = reinterpret_cast<const long long*>(c.partial);
     long long acc = 0;
     for (unsigned i = threadIdx.x; i < n_tiles; i += kFinishThreads) acc += partial[i];

@@ -943,36 +943,16 @@ template <int NT> __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-// Mass bookkeeping of one phase work item (see MassCell): publish the
-// partial, and if this is the case's last work item, turn all partials into
-// the next step's delta.
+// A stored-g phase work item's partial sum of phi (see MassCell), stored
+// for launch_massq_finish, which turns the partials into the next step's
+// delta.
 template <int TX, int TY>
 __device__ __forceinline__ void mass_partial(const CaseView& c, const WorkItem& w, double part) {
-    constexpr int NT = TX * TY;
     const Geo& G = c.geo;
     const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
-    const unsigned n_tiles = gx * gy * (unsigned)((G.nz + c.zc - 1) / c.zc);
     const unsigned tile = ((unsigned)(w.z0 / c.zc) * gy + (unsigned)(w.y0 / TY)) * gx + (unsigned)(w.x0 / TX);
-    const int tid = threadIdx.y * TX + threadIdx.x;
-    __shared__ int s_last;
-    const double s = block_sum<NT>(part);
-    if (tid == 0) {
-        __hip_atomic_store(&c.partial[tile], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned done =
-            __hip_atomic_fetch_add(&c.mass->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = done == n_tiles - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double acc = 0.0;
-    for (unsigned i = (unsigned)tid; i < n_tiles; i += NT)
-        acc += __hip_atomic_load(&c.partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double total = block_sum<NT>(acc);
-    if (tid == 0) {
-        c.mass->delta[c.mslot ^ 1] = (c.mass->target - total) / (double)G.n;
-        __hip_atomic_store(&c.mass->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const double s = block_sum<TX * TY>(part);
+    if (threadIdx.y == 0 && threadIdx.x == 0) c.partial[tile] = s;
 }
 
 template <int TX, int TY, class S>
