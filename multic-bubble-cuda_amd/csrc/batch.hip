@@ -497,7 +497,7 @@ mcb_status Batch::launch_step(int q) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_phase_batch(d_views_[q], gr.d_work, gr.n, gr.t, storage_, stream_));
     }
-    if (any_mass_) B_HIP(launch_massq_finish(d_views_[q], (unsigned)order_.size(), stream_));
+    if (any_mass_) B_HIP(launch_mass_finish(d_views_[q], (unsigned)order_.size(), stream_));
     for (auto& kv : groups_) {
         const Group& gr = kv.second;
         if (gr.n) B_HIP(launch_collide_batch(d_views_[q], gr.d_work, gr.n, gr.t, gr.mode, storage_, stream_));

@@ -34,7 +34,7 @@ struct Tiling {
 // Mass bookkeeping of a case with 16-bit populations (device memory). The
 // phase pass adds delta[q] to every phi it computes at case parity q and
 // writes one FP64 partial sum of phi per work item; after the pass
-// launch_massq_finish sums the partials in a fixed order and stores
+// launch_mass_finish sums the partials in a fixed order and stores
 // delta[q ^ 1] = (target - sum phi) / cells for the next step.
 struct MassCell {
     double target;          // sum of phi the case must keep
@@ -46,7 +46,7 @@ struct MassCell {
 // per unit), an exact and hence order-independent sum, so the total and the
 // correction do not depend on how cells fall into workgroups (the step stays
 // the same bits under shifts and tilings). Each workgroup stores its partial;
-// a small kernel after the pass (launch_massq_finish) sums them and records
+// a small kernel after the pass (launch_mass_finish) sums them and records
 // the first moment-form pass's total as the target, or sets the correction
 // delta that the next pass adds to every phi. (No workgroup waits on others:
 // an agent-scope release per workgroup would write back its XCD's L2.)
@@ -141,7 +141,7 @@ hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int s
 // After a batch's phase pass: the mass-drift corrections of its 16-bit
 // cases (MassCell for stored g, MassQ for the moment form), one workgroup
 // per case of the view array.
-hipError_t launch_massq_finish(const CaseView* cases, unsigned n_cases, hipStream_t s);
+hipError_t launch_mass_finish(const CaseView* cases, unsigned n_cases, hipStream_t s);
 hipError_t launch_phase_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,
                               int storage, hipStream_t s);
 hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, unsigned n_work, const Tiling& t,

@@ -173,9 +173,9 @@ bool tiling_supported16(const Tiling& t) {
 }
 
 namespace {
-// launch_massq_finish: one workgroup per case of the view array.
+// launch_mass_finish: one workgroup per case of the view array.
 constexpr int kFinishThreads = 256;
-__global__ __launch_bounds__(kFinishThreads) void k_massq_finish(const CaseView* __restrict__ cases) {
+__global__ __launch_bounds__(kFinishThreads) void k_mass_finish(const CaseView* __restrict__ cases) {
     const CaseView c = cases[blockIdx.x];
     const Geo& G = c.geo;
     const unsigned n_tiles = (unsigned)((G.nx + c.tx - 1) / c.tx) * (unsigned)((G.ny + c.ty - 1) / c.ty) *
@@ -205,9 +205,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_massq_finish(const CaseView*
 }
 }  // namespace
 
-hipError_t launch_massq_finish(const CaseView* cases, unsigned n_cases, hipStream_t s) {
+hipError_t launch_mass_finish(const CaseView* cases, unsigned n_cases, hipStream_t s) {
     if (n_cases == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_massq_finish, dim3(n_cases), dim3(kFinishThreads), 0, s, cases);
+    hipLaunchKernelGGL(k_mass_finish, dim3(n_cases), dim3(kFinishThreads), 0, s, cases);
     return hipGetLastError();
 }
 

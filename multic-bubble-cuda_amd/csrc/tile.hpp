@@ -446,7 +446,7 @@ template <int NT> __device__ __forceinline__ long long block_sum_q(long long v) 
 }
 
 // A moment-form phase work item's fixed-point partial sum of phi (see
-// MassQ), stored for launch_massq_finish.
+// MassQ), stored for launch_mass_finish.
 template <int TX, int TY>
 __device__ __forceinline__ void massq_partial(const CaseView& c, int x0, int y0, int z0, long long part) {
     const Geo& G = c.geo;
@@ -944,7 +944,7 @@ template <int NT> __device__ __forceinline__ double block_sum(double v) {
 }
 
 // A stored-g phase work item's partial sum of phi (see MassCell), stored
-// for launch_massq_finish, which turns the partials into the next step's
+// for launch_mass_finish, which turns the partials into the next step's
 // delta.
 template <int TX, int TY>
 __device__ __forceinline__ void mass_partial(const CaseView& c, const WorkItem& w, double part) {
