@@ -485,6 +485,28 @@ mcb_status ctx_init_bubbles_device(mcb_ctx* c, const BubbleSet& s) {
     c->initialised = true;
     return MCB_OK;
 }
+
+// Re-target a context at another case of the same lattice sides, storage,
+// boundary mode and g form, then initialise it from bubbles on the device.
+// The serving loop's sequential baseline keeps one context per lattice shape
+// this way instead of allocating and freeing one per request. Not part of
+// the C ABI.
+mcb_status ctx_reset_case(mcb_ctx* c, const mcb_params& p, const BubbleSet& s) {
+    if (!c) return MCB_ERR_INVALID_ARG;
+    if (host_validate(&p, &c->err) != MCB_OK) return MCB_ERR_INVALID_ARG;
+    if (p.nx != c->p.nx || p.ny != c->p.ny || p.nz != c->p.nz || p.storage != c->p.storage ||
+        p.bc_mode != c->p.bc_mode || (p.tau_g == 1.0f) != (c->p.tau_g == 1.0f))
+        return fail(c, MCB_ERR_INVALID_ARG, "a context is re-targeted only at a case of the same shape and form");
+    c->p.tau_f = p.tau_f;
+    c->p.tau_g = p.tau_g;
+    c->p.sigma = p.sigma;
+    c->p.sharp_c = p.sharp_c;
+    c->p.weight_set = p.weight_set;
+    c->p.moving_wall = p.moving_wall;
+    c->phys = host_phys(c->p);
+    mcb_op_consts_from_params(&c->p, &c->oc);
+    return ctx_init_bubbles_device(c, s);
+}
 }  // namespace mcb
 }  // extern "C++"
 
@@ -612,7 +634,10 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
                 // Arbitrary g populations have no moment form: from here on
                 // the context keeps g as populations (phi as its one buffer).
+                // (A state image of the moment form loaded after an earlier
+                // write keeps these buffers: allocate only what is missing.)
                 for (int b = 0; b < 2; ++b) {
+                    if (c->g[b]) continue;
                     mcb_status as = alloc(c, &c->g[b], 19 * (size_t)c->geo.plane);
                     if (as != MCB_OK) return as;
                 }
@@ -989,6 +1014,9 @@ mcb_status mcb_batch_add_state(mcb_batch* b, const void* buf, size_t bytes, int*
         b->err = "state image of another storage form than the batch's";
         return MCB_ERR_INVALID_ARG;
     }
+    // The header's parameters reach the kernels: hold them to the same checks
+    // as parameters passed by a caller.
+    if ((st = host_validate(&h.params, &b->err)) != MCB_OK) return st;
     return bstat(b, b->b->add_state(buf, bytes, case_id));
 }
 

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <tuple>
 
 #include "batch.hpp"
@@ -233,6 +234,8 @@ mcb_status Batch::add_bubbles(const mcb_params& p, const mcb_bubble* bubbles, in
 
 mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const BubbleSet* bubbles, int* id) {
     if (!id) return fail(MCB_ERR_INVALID_ARG, "null argument");
+    std::string why;
+    if (host_validate(&p, &why) != MCB_OK) return fail(MCB_ERR_INVALID_ARG, why);
     if (p.bc_mode != MCB_BC_PERIODIC) return fail(MCB_ERR_UNSUPPORTED, "batched cases are periodic");
     if (storage_ >= 0 && p.storage != storage_) return fail(MCB_ERR_UNSUPPORTED, "one storage form per batch");
     const long long P = plane_of(p);
@@ -740,7 +743,14 @@ mcb_status Batch::add_state(const void* buf, size_t bytes, int* id) {
     if (h.path != MCB_PATH_FUSED) return fail(MCB_ERR_INVALID_ARG, "state image of a staged-path context");
     mcb_params p = h.params;
     p.device = device_;
-    const std::vector<float> zero((size_t)p.nx * p.ny * p.nz, 0.0f);
+    std::string why;
+    if (host_validate(&p, &why) != MCB_OK) return fail(MCB_ERR_INVALID_ARG, "state image: " + why);
+    std::vector<float> zero;
+    try {
+        zero.assign((size_t)p.nx * p.ny * p.nz, 0.0f);
+    } catch (const std::bad_alloc&) {
+        return fail(MCB_ERR_OUT_OF_MEMORY, "host memory for the case's placeholder phi");
+    }
     int nid = 0;
     if ((st = add(p, zero.data(), &nid)) != MCB_OK) return st;
     if ((st = load_state(nid, buf, bytes)) != MCB_OK) {

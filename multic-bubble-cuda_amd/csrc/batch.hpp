@@ -62,6 +62,7 @@ struct CaseState {
 
 // Host-side constants of a parameter set (host.cpp).
 Phys host_phys(const mcb_params& p);
+mcb_status host_validate(const mcb_params* p, std::string* why);
 
 // Device bytes of a case with these parameters (populations, phase field(s)
 // and, at tau_g = 1, the moment form of g).
@@ -75,7 +76,8 @@ public:
     Batch& operator=(const Batch&) = delete;
 
     // Allocate a case and set its initial state from phi (host, nx*ny*nz
-    // floats), or from bubbles on the device (serving; device tanhf).
+    // floats), or from bubbles on the device (serving; bubble_phi_cell, the
+    // bits of the host initialisation).
     mcb_status add(const mcb_params& p, const float* phi_host, int* id);
     mcb_status add_bubbles(const mcb_params& p, const mcb_bubble* bubbles, int n, float width, int* id);
     mcb_status remove(int id);

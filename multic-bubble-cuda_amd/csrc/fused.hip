@@ -7,6 +7,7 @@
 //             row shifted by at most one cell) and one aligned store.
 // k_collide : a workgroup owns a TX x TY column of cells and marches it
 //             through zc planes (collide_tile).
+#include "bubble.hpp"
 #include "tile.hpp"
 
 namespace mcb {
@@ -188,25 +189,15 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
-// Periodic initial phase field of a set of bubbles, the same formula as the
-// host initialisation (src/init.cu:21-35 with min-image distances).
+// Periodic initial phase field of a set of bubbles: bubble_phi_cell, the
+// same bits as the host initialisation (src/init.cu:27-49 with minimum-image
+// distances).
 __global__ __launch_bounds__(PBX * PBY) void k_bubble_phi(float* __restrict__ phi, Geo G, BubbleSet s) {
     const int i = blockIdx.x * PBX + threadIdx.x;
     const int j = blockIdx.y * PBY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= G.nx || j >= G.ny) return;
-    const float hx = (float)G.nx * 0.5f, hy = (float)G.ny * 0.5f, hz = (float)G.nz * 0.5f;
-    float v = 0.0f;
-    for (int b = 0; b < s.n; ++b) {
-        float dx = (float)i - s.b[b].cx, dy = (float)j - s.b[b].cy, dz = (float)k - s.b[b].cz;
-        if (dx > hx) dx -= (float)G.nx; else if (dx < -hx) dx += (float)G.nx;
-        if (dy > hy) dy -= (float)G.ny; else if (dy < -hy) dy += (float)G.ny;
-        if (dz > hz) dz -= (float)G.nz; else if (dz < -hz) dz += (float)G.nz;
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float pb = 0.5f + 0.5f * tanhf(2.0f * (s.b[b].r - r) / s.width);
-        v = b == 0 ? pb : (pb > v ? pb : v);
-    }
-    phi[(size_t)k * G.nxy + (size_t)j * G.nx + i] = v;
+    phi[(size_t)k * G.nxy + (size_t)j * G.nx + i] = bubble_phi_cell(i, j, k, G.nx, G.ny, G.nz, true, s.b, s.n, s.width);
 }
 
 // FP64 sum of a float field: per-block partial sums in a fixed order, then

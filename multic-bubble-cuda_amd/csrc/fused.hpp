@@ -171,8 +171,8 @@ struct BubbleSet {
     int n;
     float width;
 };
-// Periodic initial phi of a bubble set on the device (the host formula,
-// device tanhf/sqrtf).
+// Periodic initial phi of a bubble set on the device (bubble_phi_cell: the
+// bits of the host initialisation).
 hipError_t launch_bubble_phi(float* phi, const Geo& geo, const BubbleSet& bubbles, hipStream_t s);
 // *out = sum of v[0..n) in FP64 (partial: kReduceBlocks doubles of scratch).
 hipError_t launch_sum(const float* v, long long n, double* partial, double* out, hipStream_t s);

@@ -13,6 +13,7 @@
 
 #include "../../include/mcb.h"
 #include "../../include/mcb_ops.h"
+#include "bubble.hpp"
 #include "lattice.hpp"
 
 namespace mcb {
@@ -124,31 +125,13 @@ mcb_status mcb_host_init_phi(const mcb_params* p, int n, const mcb_bubble* bubbl
             return MCB_ERR_INVALID_ARG;
     }
     const bool periodic = p->bc_mode == MCB_BC_PERIODIC;
-    const float hx = (float)NX * 0.5f, hy = (float)NY * 0.5f, hz = (float)NZ * 0.5f;
     for (int k = 0; k < NZ; ++k)
         for (int j = 0; j < NY; ++j)
             for (int i = 0; i < NX; ++i) {
                 const size_t idx = (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k);
                 const bool face = i == 0 || i == NX - 1 || j == 0 || j == NY - 1 || k == 0 || k == NZ - 1;
-                if (!periodic && face) {
-                    phi_out[idx] = 0.0f;   // cudaMemset(d_phi, 0) (src/var.cu:86)
-                    continue;
-                }
-                float v = 0.0f;
-                for (int b = 0; b < n; ++b) {
-                    float dx = (float)i - bubbles[b].cx;
-                    float dy = (float)j - bubbles[b].cy;
-                    float dz = (float)k - bubbles[b].cz;
-                    if (periodic) {
-                        if (dx > hx) dx -= (float)NX; else if (dx < -hx) dx += (float)NX;
-                        if (dy > hy) dy -= (float)NY; else if (dy < -hy) dy += (float)NY;
-                        if (dz > hz) dz -= (float)NZ; else if (dz < -hz) dz += (float)NZ;
-                    }
-                    const float Ri = std::sqrt(dx * dx + dy * dy + dz * dz);
-                    const float pb = 0.5f + 0.5f * std::tanh(2.0f * (bubbles[b].r - Ri) / width);
-                    v = (b == 0) ? pb : (pb > v ? pb : v);
-                }
-                phi_out[idx] = v;
+                // REFERENCE mode: faces keep cudaMemset(d_phi, 0) (src/var.cu:86).
+                phi_out[idx] = !periodic && face ? 0.0f : bubble_phi_cell(i, j, k, NX, NY, NZ, periodic, bubbles, n, width);
             }
     return MCB_OK;
 }

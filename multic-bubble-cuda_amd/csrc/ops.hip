@@ -4,6 +4,7 @@
 // 8x8x8 blocks, so every population plane is read and written in 256-byte
 // runs. The per-cell arithmetic lives in lattice.hpp and is shared with the
 // fused step kernels.
+#include "bubble.hpp"
 #include "common.hpp"
 
 namespace mcb {
@@ -60,25 +61,13 @@ struct BubbleArgs {
     float width;
 };
 
+// bubble_phi_cell: the same bits as the host initialisation.
 __global__ __launch_bounds__(256) void k_initPhase(float* phi, OpArgs a, BubbleArgs bb) {
     int i, j, k;
     if (!cell_of(a, true, i, j, k)) return;
-    float v = 0.0f;
-    for (int b = 0; b < bb.n; ++b) {
-        float dx = (float)i - bb.b[4 * b + 0];
-        float dy = (float)j - bb.b[4 * b + 1];
-        float dz = (float)k - bb.b[4 * b + 2];
-        if (a.mode == MCB_BC_PERIODIC) {
-            const float hx = (float)a.geo.nx * 0.5f, hy = (float)a.geo.ny * 0.5f, hz = (float)a.geo.nz * 0.5f;
-            if (dx > hx) dx -= (float)a.geo.nx; else if (dx < -hx) dx += (float)a.geo.nx;
-            if (dy > hy) dy -= (float)a.geo.ny; else if (dy < -hy) dy += (float)a.geo.ny;
-            if (dz > hz) dz -= (float)a.geo.nz; else if (dz < -hz) dz += (float)a.geo.nz;
-        }
-        const float Ri = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float pb = 0.5f + 0.5f * tanhf(2.0f * (bb.b[4 * b + 3] - Ri) / bb.width);
-        v = (b == 0) ? pb : fmaxf(v, pb);
-    }
-    phi[lin(i, j, k, a.geo)] = v;
+    static_assert(sizeof(mcb_bubble) == 4 * sizeof(float), "bubble record = cx, cy, cz, r");
+    phi[lin(i, j, k, a.geo)] = bubble_phi_cell(i, j, k, a.geo.nx, a.geo.ny, a.geo.nz, a.mode == MCB_BC_PERIODIC,
+                                               reinterpret_cast<const mcb_bubble*>(bb.b), bb.n, bb.width);
 }
 
 __global__ __launch_bounds__(256) void k_initDist(const float* rho, const float* phi, float* f, float* g, OpArgs a) {

@@ -187,4 +187,31 @@ double ora_time_steps(void* hp, int n) {
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// FP32 tanh of the initial profile (tanh_f32) against the C library's double
+// tanh rounded to float, for every float bit pattern in [lo, hi] (non-negative
+// floats order like their bit patterns). out[0] = values compared, out[1] =
+// results that differ, out[2] = results more than one ulp apart.
+void ora_tanh_scan(uint32_t lo, uint32_t hi, long long* out) {
+    long long n = 0, diff = 0, far = 0;
+#pragma omp parallel for reduction(+ : n, diff, far) schedule(static, 1 << 16)
+    for (long long u = (long long)lo; u <= (long long)hi; ++u) {
+        const uint32_t bits = (uint32_t)u;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        const float a = mcb::tanh_f32(x);
+        const float b = (float)std::tanh((double)x);
+        ++n;
+        if (std::memcmp(&a, &b, 4) != 0) {
+            ++diff;
+            int32_t ia, ib;
+            std::memcpy(&ia, &a, 4);
+            std::memcpy(&ib, &b, 4);
+            if (ia - ib > 1 || ib - ia > 1) ++far;
+        }
+    }
+    out[0] = n;
+    out[1] = diff;
+    out[2] = far;
+}
+
 }  // extern "C"

@@ -38,6 +38,11 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
+// tanh_f32: the library's FP32 tanh (IEEE double operations only, so host and
+// GPU give the same bits), pinned against the C library's double tanh rounded
+// to float over every float in [0, 20) by tests/test_oracle_kat.py.
+#include "../multic-bubble-cuda_amd/csrc/bubble.hpp"
 
 namespace mcbo {
 
@@ -183,7 +188,13 @@ template <class T> T bubble_phi(const State<T>& s, int i, int j, int k, const Bu
     }
     T Ri = std::sqrt(dx * dx + dy * dy + dz * dz);
     T bubble_radius = T(b.r);
-    return T(0.5) + T(0.5) * std::tanh(T(2) * (bubble_radius - Ri) / width);
+    const T arg = T(2) * (bubble_radius - Ri) / width;
+    // FP32: the reference's tanhf (init.cu:45) as tanh_f32, whose bits the
+    // host and the GPU share; FP64: the C library's tanh.
+    T th;
+    if constexpr (std::is_same_v<T, float>) th = mcb::tanh_f32(arg);
+    else th = std::tanh(arg);
+    return T(0.5) + T(0.5) * th;
 }
 
 // initPhase (init.cu:27-49): interior only in REFERENCE mode (faces keep the

@@ -72,6 +72,7 @@ def lib():
         L.ora_max_threads.restype = i
         L.ora_time_steps.restype = d
         L.ora_time_steps.argtypes = [vp, i]
+        L.ora_tanh_scan.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
 
@@ -145,6 +146,15 @@ class Oracle:
 
 def set_threads(n):
     lib().ora_set_threads(int(n))
+
+
+def tanh_scan(lo_bits, hi_bits):
+    """(compared, differing, more than 1 ulp apart): the FP32 tanh of the
+    initial profile against the C library's double tanh rounded to float, for
+    every float bit pattern in [lo_bits, hi_bits] (non-negative floats)."""
+    out = (ctypes.c_longlong * 3)()
+    lib().ora_tanh_scan(int(lo_bits), int(hi_bits), out)
+    return out[0], out[1], out[2]
 
 
 def max_threads():

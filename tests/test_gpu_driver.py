@@ -113,22 +113,6 @@ def test_nonfinite_populations_are_detected(gpu_lib, path):
     lat.close()
 
 
-def test_cfg1_geometry_bitexact_short(gpu_lib):
-    """The 128^3 seeded off-centre case, 20 steps, fused path vs oracle."""
-    mcb = gpu_lib
-    c = mcb.case_preset("cfg1", 0)
-    lat = mcb.Lattice.from_case(c)
-    o = oracle_for(c.params)
-    o.init(f32_bubbles(mcb.case_bubbles(c)), c.width)
-    lat.step(20)
-    o.step(20)
-    got = lat.observables()
-    o.macros()
-    for k in ("phi", "rho", "ux", "uy", "uz"):
-        assert np.array_equal(got[k], o.field(k)), k
-    lat.close()
-
-
 @pytest.mark.parametrize("tau_g", ["0.8", "1"])
 def test_driver_resumes_from_a_state_image(gpu_lib, tmp_path, tau_g):
     """--save-state after 20 iterations, then --load-state in a second run:

@@ -331,14 +331,51 @@ def test_openmp_is_bitwise_equal_to_serial():
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
-def test_periodic_step_conserves_phi():
-    o = O.Oracle(32, 32, 32, prec=32, tau_f=0.55, tau_g=0.8, weights=O.W_UNIT_SUM)
+def test_cfg0_conserves_phi_and_holds_a_laplace_jump():
+    """cfg0: 64^3, R = 10 at the centre, tau_f 0.55, sigma 0.1, 1000 steps.
+
+    Sum phi: the collisions conserve phi exactly in exact arithmetic; what
+    remains is FP32 rounding, which random-walks: 8e-8 relative at tau_g = 1
+    (the reference's collision) and 1.4e-7 with the tau_g = 0.8 extension,
+    whose (1 - omega_g) g + omega_g g_post rounds twice more per link. The
+    FP64 transcription keeps it to 1e-14.
+
+    Laplace jump at tau_g = 0.8: at this low viscosity pressure waves make
+    dp oscillate by +-30 % around its mean (0.81-1.67 x 2 sigma / R between
+    steps 400 and 1000); its mean over those steps is 1.34 x 2 sigma / R. At
+    R = 10 the profile of width W = 3 covers a third of the radius, so the
+    discrete curvature force overshoots the sharp-interface value; the
+    48^3 case at tau_f = 1 below meets 15 %."""
+    n, R, sigma = 64, 10.0, 0.1
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    r = np.sqrt((x - 32.0) ** 2 + (y - 32.0) ** 2 + (z - 32.0) ** 2)
+    for tau_g, bound in ((1.0, 1e-7), (0.8, 2e-7)):
+        o = O.Oracle(n, n, n, prec=32, tau_f=0.55, tau_g=tau_g, sigma=sigma, weights=O.W_UNIT_SUM)
+        o.init([(32, 32, 32, R)], 3.0)
+        o.macros()
+        s0 = o.sum_phi()
+        dps = []
+        if tau_g == 0.8:
+            o.step(400)
+            for _ in range(60):
+                o.step(10)
+                o.macros()
+                rho = o.field("rho").astype(np.float64)
+                dps.append(CS2 * (rho[r < R - 6].mean() - rho[r > R + 6].mean()))
+        else:
+            o.step(1000)
+            o.macros()
+        assert abs(o.sum_phi() - s0) / s0 < bound, tau_g
+        if dps:
+            ratio = np.mean(dps) / (2 * sigma / R)
+            assert 0.9 < ratio < 1.6, ratio
+    o = O.Oracle(32, 32, 32, prec=64, tau_f=0.55, tau_g=0.8, weights=O.W_UNIT_SUM)
     o.init([(16, 16, 16, 7)], 3.0)
     o.macros()
     s0 = o.sum_phi()
     o.step(200)
     o.macros()
-    assert abs(o.sum_phi() - s0) / s0 < 1e-7
+    assert abs(o.sum_phi() - s0) / s0 < 1e-14
 
 
 def test_literal_weights_gain_mass_at_two_pow_minus_26_per_step():
