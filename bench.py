@@ -11,9 +11,10 @@ HIP events around each launch inside it give the per-kernel durations for
 the roofline. Prints one JSON line.
 
 Beside the plain line (unless --plain-only): the same case with FP16 and
-BF16 population storage; the uneven 64-case batch advanced together vs one
-case at a time; and the serving trace (Poisson arrivals, FP16 continuous
-batching) vs the plain FP32 path serving one request at a time.
+BF16 population storage; the 128^3 tau_g = 0.8 case with g stored as
+populations; the uneven 64-case batch advanced together vs one case at a
+time; and the serving trace (Poisson arrivals, FP16 continuous batching) vs
+the plain FP32 path serving one request at a time.
 """
 import argparse
 import json
@@ -26,14 +27,33 @@ sys.path.insert(0, os.path.join(ROOT, "multic-bubble-cuda_amd"))
 
 HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E peak
 B_ALG_STEP = 2 * 19 * 2 * 4           # f and g, 19 pops, read + write, FP32: 304 B per lattice update
-# Stored g populations (tau_g != 1):
-B_ALG_COLLIDE = 3 * 19 * 4            # k_collide: f read, f write, g write: 228 B per cell
-B_ALG_PHASE = 19 * 4                  # k_phase: g read: 76 B per cell
+# Stored g populations (tau_g != 1): the BGK phase collision reads the
+# pre-collision g (the gathered post-stream populations) besides writing g*.
+B_ALG_COLLIDE = 4 * 19 * 4 + 4        # k_collide: f read + write, g read + write, phi read: 308 B per cell
+B_ALG_PHASE = 19 * 4 + 4              # k_phase: g read, phi write: 80 B per cell
 # g in moment form (tau_g = 1, the benchmark case): g*_l(y) is recomputed from
 # phi, u, n of its source cell instead of being stored.
 B_ALG_COLLIDE_GM = 2 * 19 * 4 + 6 * 4 + 4   # f read, f write, u and n write, phi read: 180 B per cell
 B_ALG_PHASE_GM = 7 * 4 + 4                  # phi, u, n read; phi write: 32 B per cell
 METRIC = "MLUPS at 256³ + % of HBM roofline; serving p50/p99 latency vs plain solves"
+
+
+def host_cpu():
+    """CPU model and core counts of this host (for the CPU baseline record)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count()}
 
 
 def cpu_baseline(case, seconds_target=15.0):
@@ -52,8 +72,41 @@ def cpu_baseline(case, seconds_target=15.0):
     t = o.time_steps(n)
     threads = oracle.max_threads()
     o.close()
-    return {"value": round(cells * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the same 256^3 case after 1 warm-up step, OpenMP over {threads} threads"}
+    out = {"value": round(cells * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"{n} steps of the same 256^3 case after 1 warm-up step, OpenMP over {threads} threads"}
+    out.update(host_cpu())
+    return out
+
+
+def stored_g(mcb, steps, warmup, seed=0):
+    """cfg1: 128^3, seeded off-centre bubble, tau_f 0.55, tau_g 0.8, FP32.
+    With tau_g != 1 the phase collision relaxes towards the reference's
+    equilibrium (BGK), so g is kept as 19 populations: k_phase sums the 19
+    gathered g (80 B per cell), k_collide reads f and the gathered g, writes
+    f* and g* (308 B per cell): 388 B moved per lattice update against the
+    304 B of the population roofline."""
+    case = mcb.case_preset("cfg1", seed)
+    p = case.params
+    cells = p.nx * p.ny * p.nz
+    lat = mcb.Lattice.from_case(case, path=mcb.PATH_FUSED)
+    lat.step(warmup)
+    t_phase, t_collide = lat.kernel_times(steps)
+    ms = lat.step_timed(steps) / steps
+    lat.observables()
+    s, k, finite = lat.reduce()
+    tiling = lat.tiling()
+    lat.close()
+    mlups = cells / (ms * 1e-3) / 1e6
+    ph, co = t_phase / steps, t_collide / steps
+    return {"workload": "128^3 periodic D3Q19 f+g, seeded off-centre bubble R in [12,24] (seed %d), sigma 0.1, "
+                        "tau_f 0.55, tau_g 0.8, FP32, g as populations" % seed,
+            "mlups": round(mlups, 1), "ms_per_step": round(ms, 4), "tiling": list(tiling),
+            "roofline_frac_304": round(mlups * 1e6 * B_ALG_STEP / 1e9 / HBM_PEAK_GBS, 4),
+            "b_moved_per_lu": B_ALG_COLLIDE + B_ALG_PHASE,
+            "moved_frac": round(mlups * 1e6 * (B_ALG_COLLIDE + B_ALG_PHASE) / 1e9 / HBM_PEAK_GBS, 4),
+            "k_phase_ms": round(ph, 5), "k_phase_gbs": round(B_ALG_PHASE * cells / (ph * 1e-3) / 1e9, 1),
+            "k_collide_ms": round(co, 5), "k_collide_gbs": round(B_ALG_COLLIDE * cells / (co * 1e-3) / 1e9, 1),
+            "sum_phi": s, "finite": finite}
 
 
 def storage_form(mcb, case, storage, steps, warmup):
@@ -164,6 +217,8 @@ def main():
                     help="moderate serving arrival rate, requests per second (latency section)")
     ap.add_argument("--form16", choices=("fp16", "bf16"), default=None,
                     help="only time the 256^3 case with this 16-bit storage form (profiling)")
+    ap.add_argument("--stored-g", action="store_true",
+                    help="only time the 128^3 tau_g = 0.8 case (g as populations; profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (byte-counter calibration)")
     args = ap.parse_args()
@@ -183,6 +238,9 @@ def main():
         case = mcb.case_preset("cfg2", args.seed)
         st = mcb.STORE_FP16 if args.form16 == "fp16" else mcb.STORE_BF16
         print(json.dumps({args.form16: storage_form(mcb, case, st, args.steps, args.warmup)}), flush=True)
+        return
+    if args.stored_g:
+        print(json.dumps({"stored_g_128": stored_g(mcb, args.steps, args.warmup)}), flush=True)
         return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
@@ -260,6 +318,7 @@ def main():
         for name, st in (("fp16_storage", mcb.STORE_FP16), ("bf16_storage", mcb.STORE_BF16)):
             out[name] = storage_form(mcb, case, st, args.steps, args.warmup)
             out[name]["vs_plain"] = round(out[name]["mlups"] / out["value"], 3)
+        out["stored_g_128"] = stored_g(mcb, args.steps, args.warmup)
         out["batching"] = batching(mcb)
         out["serving"] = serving(mcb, step_scale=args.serve_scale, moderate_rate=args.serve_rate)
     if not args.no_cpu_baseline:

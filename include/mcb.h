@@ -57,7 +57,9 @@ typedef enum mcb_storage { MCB_STORE_FP32 = 0, MCB_STORE_FP16 = 1, MCB_STORE_BF1
 
 /* Kernel path. FUSED = 2 passes per step (phase recovery, then
  * normals/curvature/moments/collisions fused with LDS-staged stencils);
- * STAGED = the reference's per-kernel decomposition (mcb_ops.h kernels). */
+ * STAGED = the reference's per-kernel decomposition (mcb_ops.h kernels).
+ * AUTO picks FUSED for PERIODIC cases, and for REFERENCE cases in FP32 at
+ * tau_g = 1 without moving_wall (the reference preset); STAGED otherwise. */
 typedef enum mcb_path { MCB_PATH_AUTO = 0, MCB_PATH_FUSED = 1, MCB_PATH_STAGED = 2 } mcb_path;
 
 typedef struct mcb_params {
@@ -85,7 +87,10 @@ typedef struct mcb_bubble {
  * iteration (REFERENCE mode: with boundaryConditions applied). RHO, UX, UY,
  * UZ, FX, FY, FZ, NORMX, NORMY, NORMZ hold the values of the most recent
  * mcb_compute_macros call. F and G (19 planes, population-major, x fastest)
- * are the pre-collision populations in the reference layout (src/header.cuh:21). */
+ * are the pre-collision populations in the reference layout (src/header.cuh:21).
+ * On the FUSED path with REFERENCE boundaries, G is neither read nor written
+ * and PHI is not written (MCB_ERR_UNSUPPORTED; the STAGED path keeps every
+ * reference buffer); F is read and written exactly. */
 typedef enum mcb_field {
     MCB_FIELD_PHI = 0, MCB_FIELD_RHO = 1, MCB_FIELD_UX = 2, MCB_FIELD_UY = 3, MCB_FIELD_UZ = 4,
     MCB_FIELD_FX = 5, MCB_FIELD_FY = 6, MCB_FIELD_FZ = 7,

@@ -69,6 +69,10 @@ struct mcb_ctx {
     float* gmu[3] = {nullptr, nullptr, nullptr};
     float* gmn[3] = {nullptr, nullptr, nullptr};
     long long steps = 0;   // steps since the initial state (non-batch paths)
+    // Fused path with REFERENCE boundaries: a written F put arbitrary values
+    // into the face populations of f[ref_reset_f], which the next step reads
+    // once; after it they are reset to w_l (-1: nothing pending).
+    int ref_reset_f = -1;
 };
 
 struct mcb_batch {
@@ -99,9 +103,24 @@ mcb_status hip_fail(mcb_ctx* c, hipError_t e, const char* where) {
         if (e_ != hipSuccess) return hip_fail((c), e_, #call);    \
     } while (0)
 
+// boundaryConditions (src/lbm.cu:403-425, faces in the order z, y, x) on a
+// host phi field: every face cell takes the phi of the interior cell at its
+// clamped coordinates.
+void reference_faces_from_interior(const mcb_params& p, float* phi) {
+    const int NX = p.nx, NY = p.ny, NZ = p.nz;
+    auto idx = [&](int i, int j, int k) { return (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k); };
+    auto cl = [](int a, int n) { return a < 1 ? 1 : (a > n - 2 ? n - 2 : a); };
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j) {
+            const bool edge = k == 0 || k == NZ - 1 || j == 0 || j == NY - 1;
+            for (int i = 0; i < NX; i += (edge || i == NX - 1) ? 1 : NX - 1)
+                phi[idx(i, j, k)] = phi[idx(cl(i, NX), cl(j, NY), cl(k, NZ))];
+        }
+}
+
 // Plane stride: cells rounded up to 256 elements, so every population
 // plane starts 1 KiB-aligned.
-long long plane_stride(long long n) { return (n + 255) / 256 * 256; }
+long long plane_stride(long long n) { return plane_stride_cells(n); }
 
 // The fused kernels address all 19 planes of a buffer through one 32-bit
 // buffer resource (FP32 lattices up to ~56.5M cells, 16-bit up to ~113M).
@@ -110,11 +129,19 @@ bool fused_fits(const mcb_params& p) {
     return P * 19 * (p.storage == MCB_STORE_FP32 ? 4 : 2) <= 0xFFFFFFFFLL;
 }
 
+// The fused kernels take the reference's ghost-layer boundaries in FP32 with g
+// in moment form (tau_g = 1, the reference's own phase collision) and without
+// the dead movingWallBC; other REFERENCE cases run the staged kernels.
+bool fused_reference_ok(const mcb_params& p) {
+    return p.bc_mode == MCB_BC_REFERENCE && p.storage == MCB_STORE_FP32 && p.tau_g == 1.0f && !p.moving_wall;
+}
+
 int resolve_path(const mcb_params& p) {
     if (p.path == MCB_PATH_STAGED) return MCB_PATH_STAGED;
     if (p.path == MCB_PATH_FUSED) return MCB_PATH_FUSED;
     if (p.storage != MCB_STORE_FP32) return MCB_PATH_FUSED;
-    return p.bc_mode == MCB_BC_PERIODIC && fused_fits(p) ? MCB_PATH_FUSED : MCB_PATH_STAGED;
+    return (p.bc_mode == MCB_BC_PERIODIC || fused_reference_ok(p)) && fused_fits(p) ? MCB_PATH_FUSED
+                                                                                    : MCB_PATH_STAGED;
 }
 
 size_t footprint(const mcb_params& p, int path) {
@@ -149,6 +176,7 @@ CaseView fused_view(mcb_ctx* c) {
     v.gdst = c->g[c->cur ^ 1];
     v.phi = c->phi;
     v.gm = c->gm;
+    v.bc = c->p.bc_mode;
     if (c->gm != GM_POPULATIONS) {
         v.gsrc = v.gdst = nullptr;
         v.phi = c->phis[c->cur];
@@ -170,6 +198,7 @@ int step_mode(const mcb_ctx* c) { return c->p.tau_g == 1.0f ? COLLIDE_STEP : COL
 // form phi0 goes to the state buffer phis[1].
 void fused_init_target(mcb_ctx* c) {
     c->cur = 0;
+    c->ref_reset_f = -1;
     if (c->g[0] == nullptr) {
         c->gm = GM_INITIAL;
         c->phi = c->phis[1];
@@ -191,19 +220,27 @@ hipError_t fused_init_state(mcb_ctx* c) {
 
 // After a fused step: flip the A/B buffers; in moment form the phi just
 // written becomes the state's phi.
-void after_fused_step(mcb_ctx* c) {
+hipError_t after_fused_step(mcb_ctx* c) {
+    hipError_t e = hipSuccess;
+    if (c->ref_reset_f == c->cur) {
+        // The step just read the written faces; the next one writes this
+        // buffer's interior, so w_l everywhere restores the face invariant.
+        e = launch_init_pull(c->phi, c->f[c->cur], nullptr, c->geo, c->phys, c->stream);
+        c->ref_reset_f = -1;
+    }
     c->cur ^= 1;
     if (c->gm != GM_POPULATIONS) {
         c->gm = GM_MOMENTS;
         c->phi = c->phis[c->cur ^ 1];
     }
+    return e;
 }
 
 mcb_status fused_step(mcb_ctx* c) {
     const CaseView v = fused_view(c);
     MCB_HIP(c, launch_phase(v, c->tiling, c->stream));
     MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
-    after_fused_step(c);
+    MCB_HIP(c, after_fused_step(c));
     return MCB_OK;
 }
 
@@ -354,7 +391,8 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
     mcb_status st = host_validate(p, &why);
     if (st != MCB_OK) return st;
     const int path = resolve_path(*p);
-    if (path == MCB_PATH_FUSED && (p->bc_mode != MCB_BC_PERIODIC || !fused_fits(*p))) return MCB_ERR_UNSUPPORTED;
+    if (path == MCB_PATH_FUSED && ((p->bc_mode != MCB_BC_PERIODIC && !fused_reference_ok(*p)) || !fused_fits(*p)))
+        return MCB_ERR_UNSUPPORTED;
     if (p->storage != MCB_STORE_FP32 && path != MCB_PATH_FUSED) return MCB_ERR_UNSUPPORTED;
 
     int count = 0;
@@ -409,6 +447,20 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
             float** gmf[] = {&c->phis[1], &c->gmu[0], &c->gmu[1], &c->gmu[2], &c->gmn[0], &c->gmn[1], &c->gmn[2]};
             for (auto* q : gmf)
                 if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
+        }
+        if (p->bc_mode == MCB_BC_REFERENCE) {
+            // Face cells never collide: their f*_l stay w_l in both buffers
+            // (fgBoundary's rho(e) w_l with the face rho of 1) and their u and
+            // n stay 0, so the moment form gives w_l phi(e) for their g.
+            for (int b = 0; b < 2; ++b)
+                if (launch_init_pull(c->phi, c->f[b], nullptr, c->geo, c->phys, c->stream) != hipSuccess)
+                    return bail(MCB_ERR_HIP);
+            for (int k = 0; k < 3; ++k) {
+                if (hipMemsetAsync(c->gmu[k], 0, (size_t)P * sizeof(float), c->stream) != hipSuccess ||
+                    hipMemsetAsync(c->gmn[k], 0, (size_t)P * sizeof(float), c->stream) != hipSuccess)
+                    return bail(MCB_ERR_HIP);
+            }
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MCB_ERR_HIP);
         }
         c->tiling = choose_tiling(*p);
     } else {
@@ -596,6 +648,9 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
             const float* src = field == MCB_FIELD_F ? c->sf : c->sg;
             MCB_HIP(c, hipMemcpyAsync(host, src, 19 * cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         } else {
+            if (field == MCB_FIELD_G && c->p.bc_mode == MCB_BC_REFERENCE)
+                return fail(c, MCB_ERR_UNSUPPORTED,
+                            "g in the reference layout with ghost-layer boundaries: use the staged path");
             // Gather into the idle A/B buffer, then copy plane by plane.
             if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
                 MCB_HIP(c, launch_gm_to_ref(fused_view(c), c->f[c->cur ^ 1], c->stream));
@@ -608,12 +663,18 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
             for (int l = 0; l < 19; ++l)
                 MCB_HIP(c, hipMemcpyAsync(host + l * cells, tmp + (size_t)l * c->geo.plane, cells * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream));
+            if (c->p.bc_mode == MCB_BC_REFERENCE)   // restore the idle buffer's face populations (w_l)
+                MCB_HIP(c, launch_init_pull(c->phi, c->f[c->cur ^ 1], nullptr, c->geo, c->phys, c->stream));
         }
     } else {
         const float* src = scalar_field(c, field);
         if (!src) return fail(c, MCB_ERR_INVALID_ARG, "unknown field");
         if (n != cells) return fail(c, MCB_ERR_INVALID_ARG, "field buffer must hold nx*ny*nz floats");
         MCB_HIP(c, hipMemcpyAsync(host, src, cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        MCB_HIP(c, hipStreamSynchronize(c->stream));
+        if (field == MCB_FIELD_PHI && c->path == MCB_PATH_FUSED && c->p.bc_mode == MCB_BC_REFERENCE &&
+            c->gm == GM_MOMENTS && src == c->phis[c->cur ^ 1])
+            reference_faces_from_interior(c->p, host);
     }
     MCB_HIP(c, hipStreamSynchronize(c->stream));
     return MCB_OK;
@@ -631,6 +692,8 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             float* dst = field == MCB_FIELD_F ? c->sf : c->sg;
             MCB_HIP(c, hipMemcpyAsync(dst, host, 19 * cells * sizeof(float), hipMemcpyHostToDevice, c->stream));
         } else {
+            if (field == MCB_FIELD_G && c->p.bc_mode == MCB_BC_REFERENCE)
+                return fail(c, MCB_ERR_UNSUPPORTED, "writing g with ghost-layer boundaries: use the staged path");
             if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
                 // Arbitrary g populations have no moment form: from here on
                 // the context keeps g as populations (phi as its one buffer).
@@ -650,8 +713,17 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
                 MCB_HIP(c, hipMemcpyAsync(tmp + (size_t)l * c->geo.plane, host + l * cells, cells * sizeof(float),
                                           hipMemcpyHostToDevice, c->stream));
             MCB_HIP(c, launch_link_shift(tmp, dst, c->geo, -1, c->stream));
+            if (c->p.bc_mode == MCB_BC_REFERENCE) {
+                // The written face populations feed the next step once (as
+                // the reference's first momentiCalc reads them); after it they
+                // go back to w_l. The idle buffer was the staging area.
+                MCB_HIP(c, launch_init_pull(c->phi, tmp, nullptr, c->geo, c->phys, c->stream));
+                c->ref_reset_f = c->cur;
+            }
         }
     } else {
+        if (c->path == MCB_PATH_FUSED && c->p.bc_mode == MCB_BC_REFERENCE && field == MCB_FIELD_PHI)
+            return fail(c, MCB_ERR_UNSUPPORTED, "writing phi with ghost-layer boundaries: use the staged path");
         if (field == MCB_FIELD_PHI && c->gm != GM_POPULATIONS && n == cells) {
             // phi is part of the moment-form state: write the buffer the next
             // phase pass recomputes, as writing phi does with populations.
@@ -713,7 +785,7 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
         MCB_HIP(c, hipEventRecord(e[1], c->stream));
         MCB_HIP(c, launch_collide(v, step_mode(c), c->tiling, c->stream));
         MCB_HIP(c, hipEventRecord(e[2], c->stream));
-        after_fused_step(c);
+        MCB_HIP(c, after_fused_step(c));
         ++c->steps;
     }
     MCB_HIP(c, hipEventSynchronize(c->tev[need - 1]));
@@ -974,6 +1046,7 @@ mcb_status mcb_load_state(mcb_ctx* c, const void* buf, size_t bytes) {
             c->phi = c->phis[c->cur ^ 1];
         }
         c->gm = h.gm;
+        c->ref_reset_f = -1;   // the image's face populations are w_l
     }
     if ((st = state_image_load(c->stream, buf, bytes, ctx_state_parts(c), &c->err)) != MCB_OK) return st;
     if (c->path == MCB_PATH_FUSED && c->gm != GM_POPULATIONS) c->phi = c->phis[c->cur ^ 1];

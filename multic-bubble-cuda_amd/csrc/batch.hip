@@ -15,8 +15,7 @@ namespace mcb {
 namespace {
 
 long long plane_of(const mcb_params& p) {
-    const long long n = (long long)p.nx * p.ny * p.nz;
-    return (n + 255) / 256 * 256;
+    return plane_stride_cells((long long)p.nx * p.ny * p.nz);
 }
 
 unsigned pop_bytes(int storage) { return storage == MCB_STORE_FP32 ? 4u : 2u; }

@@ -119,6 +119,13 @@ struct Field {
     }
 };
 
+// Population plane stride (cells) of an n-cell lattice: n rounded up to 256
+// cells, so every plane starts 1 KiB-aligned, plus MCB_PLANE_SKEW cells.
+#ifndef MCB_PLANE_SKEW
+#define MCB_PLANE_SKEW 0
+#endif
+inline long long plane_stride_cells(long long n) { return (n + 255) / 256 * 256 + MCB_PLANE_SKEW; }
+
 // Largest plane stride (cells) the fused kernels address through Planes.
 constexpr long long kMaxFusedPlane = 0xFFFFFFFFLL / (19 * 4);
 

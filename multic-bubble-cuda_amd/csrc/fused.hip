@@ -42,7 +42,13 @@ __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, un
     const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
                             (unsigned)(shift<1>(z, G.nz) * G.nxy)};
     const PopPlanes<StoreF32> gp(c.gsrc, (unsigned)G.plane);
-    st32(c.phi, zs[1] + ys[1] + xs[1], phase_sum<StoreF32>(c, gp, xs, ys, zs));
+    const unsigned me = zs[1] + ys[1] + xs[1];
+    float v = phase_sum<StoreF32>(c, gp, xs, ys, zs);
+    // Ghost-layer faces (moment form): before the first step a face keeps its
+    // initial phi, later it takes boundaryConditions' copy.
+    if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z))
+        v = ld32(c.phi_prev, c.gm == GM_INITIAL ? me : ref_clamped(G, x, y, z));
+    st32(c.phi, me, v);
 }
 
 // Moment-form phase pass (phase_tile_gm): the collide pass's tiles and grid.

@@ -88,7 +88,17 @@ struct CaseView {
     float* gm_u[3];
     float* gm_n[3];
     int gm;                 // GM_POPULATIONS, GM_MOMENTS or GM_INITIAL
-    int reserved2;
+    // MCB_BC_PERIODIC, or MCB_BC_REFERENCE (FP32 moment form of a single
+    // case): the reference's ghost-layer boundaries (src/lbm.cu:365-425) in
+    // pull form. Face cells never collide; their f*_l stays w_l and their u
+    // and n stay 0, so an interior cell that gathers from a face e receives
+    // rho(e) w_l = w_l (fgBoundary's f; face rho stays 1) and w_l phi(e)
+    // (fgBoundary's g). The phase pass writes the state phi of a face cell as
+    // boundaryConditions does: the previous state phi at the clamped interior
+    // cell (at the first step: its own initial value), so the phi buffer a
+    // collide pass reads carries the faces of the previous iteration, as the
+    // reference's d_phi does when gradCalc runs.
+    int bc;
     // COLLIDE_MACROS outputs
     float* rho;
     float* ux;

@@ -51,6 +51,20 @@ __device__ __forceinline__ int wrapmod(int v, int n) {
     return v < 0 ? v + n : v;
 }
 
+// REFERENCE boundaries (CaseView::bc): whether (x, y, z) is a face cell (or
+// lies beyond one), and the interior cell boundaryConditions copies a face
+// cell's phi from (src/lbm.cu:403-425 run z faces, then y, then x: every
+// coordinate clamped to [1, n - 2]).
+__device__ __forceinline__ bool ref_face(const Geo& G, int x, int y, int z) {
+    return x <= 0 || x >= G.nx - 1 || y <= 0 || y >= G.ny - 1 || z <= 0 || z >= G.nz - 1;
+}
+__device__ __forceinline__ unsigned ref_clamped(const Geo& G, int x, int y, int z) {
+    x = min(max(x, 1), G.nx - 2);
+    y = min(max(y, 1), G.ny - 2);
+    z = min(max(z, 1), G.nz - 2);
+    return (unsigned)z * (unsigned)G.nxy + (unsigned)(y * G.nx + x);
+}
+
 template <int C, class T> __device__ __forceinline__ T pick(T m, T z, T p) {
     if constexpr (C < 0) return m;
     else if constexpr (C > 0) return p;
@@ -412,6 +426,8 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
                 constexpr int l = decltype(lc)::value;
                 sum += s_g[(e0 - CY[l] * HX - CX[l]) * Q + l];
             });
+            // Ghost-layer faces take boundaryConditions' copy instead.
+            if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z)) sum = ld32(c.phi_prev, ref_clamped(G, x, y, z));
             st32(c.phi, (unsigned)z * (unsigned)G.nxy + own, sum);
         }
         __syncthreads();   // s_g is rewritten by the next plane
@@ -790,8 +806,14 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                     constexpr int l = decltype(lc)::value;
                     return pick<CZ[l]>(pm, p0, pp)[cc + CY[l] * PX + CX[l]];
                 };
-                if constexpr (S::kAnchored) fast::grad_normal(nb, k16, nx, ny, nz, ind);
-                else grad_normal(nb, d, nx, ny, nz, ind);
+                if constexpr (S::kAnchored) {
+                    fast::grad_normal(nb, k16, nx, ny, nz, ind);
+                } else if (c.bc == MCB_BC_REFERENCE && ref_face(G, x0 - 1 + hx, y0 - 1 + hy, z)) {
+                    // gradCalc updates interior cells only: a face keeps n = 0.
+                    nx = ny = nz = ind = 0.0f;
+                } else {
+                    grad_normal(nb, d, nx, ny, nz, ind);
+                }
                 s_n[sl][0][e] = nx;
                 s_n[sl][1][e] = ny;
                 s_n[sl][2][e] = nz;
@@ -845,6 +867,18 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
         __syncthreads();
         if (z + 1 < ze) fetch_phi(z + 3);
         if (!valid) return;
+        if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z)) {
+            // Faces never collide (interior guards of src/lbm.cu); their
+            // macroscopic fields keep the values the reference never
+            // updates: rho = 1 (initTensor), u, F and n = 0.
+            if constexpr (MODE == COLLIDE_MACROS) {
+                const unsigned fb = (unsigned)(G.n * 4), z4 = (unsigned)(z * G.nxy) * 4u;
+                Field(c.rho, fb).st(me4, z4, 1.0f);
+                float* const zero[] = {c.ux, c.uy, c.uz, c.fx, c.fy, c.fz, c.nx, c.ny, c.nz};
+                for (float* q : zero) Field(q, fb).st(me4, z4, 0.0f);
+            }
+            return;
+        }
 
         // Wave-uniform plane offsets (elements) of z-1, z, z+1.
         const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),

@@ -96,10 +96,14 @@ def test_create_without_device_reports_status():
 
 
 def test_unsupported_combinations():
+    """The fused kernels take ghost-layer boundaries only in FP32 with g in
+    moment form (tau_g = 1) and without the dead movingWallBC; these
+    combinations are refused before any device is touched."""
     L = mcb.lib()
     h = ctypes.c_void_p()
-    p = mcb.make_params(16, 16, 16, bc_mode=mcb.BC_REFERENCE, path=mcb.PATH_FUSED)
-    assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED
+    for kw in (dict(tau_g=0.8), dict(moving_wall=True), dict(storage=mcb.STORE_FP16)):
+        p = mcb.make_params(16, 16, 16, bc_mode=mcb.BC_REFERENCE, path=mcb.PATH_FUSED, **kw)
+        assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED, kw
 
 
 def test_footprint():
