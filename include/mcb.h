@@ -138,6 +138,21 @@ mcb_status mcb_reduce(mcb_ctx* ctx, double* sum_phi, double* kinetic);
 
 mcb_status mcb_synchronize(mcb_ctx* ctx);
 
+/* The FP64 diagnostics of mcb_reduce (sum of phi, kinetic energy
+ * sum(0.5*rho*|u|^2), same kernels, same fixed-order tree, same bits) over
+ * caller-owned device fields of NX*NY*NZ floats, written as two doubles to
+ * the device buffer out; asynchronous on stream (a hipStream_t or NULL), on
+ * the current device. It allocates nothing: the per-block partial sums sit
+ * in a 16 KiB array that the library holds on every device it runs on until
+ * the process ends. No counterpart in the reference (it computes no
+ * diagnostics); it is for callers that cannot allocate device memory, such
+ * as code under hipGraph capture or serving loops that forbid allocation.
+ * Calls that overlap on one device share that array and race: order them.
+ * Lattices of 2^30 cells or more are rejected (MCB_ERR_INVALID_ARG), as for
+ * every context. */
+mcb_status mcb_reduce_fields_noalloc(const float* phi, const float* rho, const float* ux, const float* uy,
+                                     const float* uz, int NX, int NY, int NZ, double* out, void* stream);
+
 /* Device time of each pass of the fused step, summed over nsteps steps
  * (hipEvents on the context stream around each launch):
  * out_ms[0] = phase recovery (k_phase), out_ms[1] = collision (k_collide).

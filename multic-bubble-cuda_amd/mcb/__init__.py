@@ -177,6 +177,7 @@ PROTOTYPES = {
     "mcb_serve_run": (_i, [ctypes.POINTER(ServeConfig), ctypes.POINTER(Request), _i, ctypes.POINTER(RequestResult),
                            ctypes.POINTER(ServeStats)]),
     "mcb_serve_trace": (_i, [ctypes.c_uint64, _i, _d, _d, ctypes.POINTER(Request)]),
+    "mcb_reduce_fields_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
 }
 # mcb_ops.h: reference kernel names -> number of device pointer arguments.
 OP_POINTERS = {
