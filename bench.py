@@ -12,9 +12,11 @@ the roofline. Prints one JSON line.
 
 Beside the plain line (unless --plain-only): the same case with FP16 and
 BF16 population storage; the 128^3 tau_g = 0.8 case with g stored as
-populations; the uneven 64-case batch advanced together vs one case at a
-time; and the serving trace (Poisson arrivals, FP16 continuous batching) vs
-the plain FP32 path serving one request at a time.
+populations; the reference preset (64x64x256, ghost-layer boundaries) on the
+fused and the staged kernels; the uneven 64-case batch (2000 steps each)
+advanced together vs one case at a time; and the serving trace (Poisson
+arrivals, FP16 continuous batching) vs the plain FP32 path serving one
+request at a time.
 """
 import argparse
 import json
@@ -106,7 +108,28 @@ def stored_g(mcb, steps, warmup, seed=0):
             "moved_frac": round(mlups * 1e6 * (B_ALG_COLLIDE + B_ALG_PHASE) / 1e9 / HBM_PEAK_GBS, 4),
             "k_phase_ms": round(ph, 5), "k_phase_gbs": round(B_ALG_PHASE * cells / (ph * 1e-3) / 1e9, 1),
             "k_collide_ms": round(co, 5), "k_collide_gbs": round(B_ALG_COLLIDE * cells / (co * 1e-3) / 1e9, 1),
+            "k_collide_traffic": traffic_for("cfg1-128-fp32-storedg"),
             "sum_phi": s, "finite": finite}
+
+
+def reference_preset(mcb, steps, warmup):
+    """The reference's own case (src/var.cu:3-4, src/init.cu:39, src/main.cu:35):
+    64x64x256 with ghost-layer boundaries, tau 0.505, sigma 0.1, one R = 20
+    bubble at the centre, FP32 with the literal weights. The fused path and the
+    staged kernels (the reference's decomposition) give the same bits."""
+    case = mcb.case_preset("reference")
+    p = case.params
+    cells = p.nx * p.ny * p.nz
+    out = {"workload": "the reference preset: 64x64x256, ghost-layer boundaries, tau 0.505, sigma 0.1, "
+                       "one R = 20 bubble at the centre, FP32"}
+    for name, path in (("fused", mcb.PATH_FUSED), ("staged", mcb.PATH_STAGED)):
+        lat = mcb.Lattice.from_case(case, path=path)
+        lat.step(warmup)
+        ms = lat.step_timed(steps) / steps
+        lat.close()
+        out[name] = {"mlups": round(cells / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 4)}
+    out["fused_vs_staged"] = round(out["fused"]["mlups"] / out["staged"]["mlups"], 3)
+    return out
 
 
 def storage_form(mcb, case, storage, steps, warmup):
@@ -126,7 +149,7 @@ def storage_form(mcb, case, storage, steps, warmup):
             "sum_phi": s, "finite": finite}
 
 
-def batching(mcb, n_cases=64, steps=100, seed=1000):
+def batching(mcb, n_cases=64, steps=2000, seed=1000):
     """The uneven batch: n_cases FP32 cases (sides 32..128) advanced together
     (one launch per pass per tile shape, graph-replayed) vs the same cases
     advanced one at a time on single-case contexts."""
@@ -156,8 +179,8 @@ def batching(mcb, n_cases=64, steps=100, seed=1000):
             "mlups_batched": round(agg, 1), "mlups_one_at_a_time": round(seq, 1),
             "speedup": round(agg / seq, 3), "device_bytes": nbytes,
             "workload": "%d 'batch' presets (seeds %d..%d): cube sides in {32,48,64,96,128}, R in [0.15,0.3] side, "
-                        "sigma in [0.01,0.1], tau_f in [0.52,0.8], FP32; %d timed steps (the preset's 2000 "
-                        "shortened for the benchmark)" % (n_cases, seed, seed + n_cases - 1, steps)}
+                        "sigma in [0.01,0.1], tau_f in [0.52,0.8], FP32; %d timed steps each"
+                        % (n_cases, seed, seed + n_cases - 1, steps)}
 
 
 def serving(mcb, n=500, step_scale=0.1, seed=0, saturated_rate=1000.0, moderate_rate=20.0):
@@ -219,6 +242,10 @@ def main():
                     help="only time the 256^3 case with this 16-bit storage form (profiling)")
     ap.add_argument("--stored-g", action="store_true",
                     help="only time the 128^3 tau_g = 0.8 case (g as populations; profiling)")
+    ap.add_argument("--reference-only", action="store_true",
+                    help="only time the reference preset on the fused and staged kernels (profiling)")
+    ap.add_argument("--batching-only", action="store_true",
+                    help="only the 64-case batch section (profiling); --steps sets its step count")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (byte-counter calibration)")
     args = ap.parse_args()
@@ -241,6 +268,12 @@ def main():
         return
     if args.stored_g:
         print(json.dumps({"stored_g_128": stored_g(mcb, args.steps, args.warmup)}), flush=True)
+        return
+    if args.reference_only:
+        print(json.dumps({"reference_preset": reference_preset(mcb, args.steps, args.warmup)}), flush=True)
+        return
+    if args.batching_only:
+        print(json.dumps({"batching": batching(mcb, steps=args.steps)}), flush=True)
         return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
@@ -319,6 +352,7 @@ def main():
             out[name] = storage_form(mcb, case, st, args.steps, args.warmup)
             out[name]["vs_plain"] = round(out[name]["mlups"] / out["value"], 3)
         out["stored_g_128"] = stored_g(mcb, args.steps, args.warmup)
+        out["reference_preset"] = reference_preset(mcb, args.steps, args.warmup)
         out["batching"] = batching(mcb)
         out["serving"] = serving(mcb, step_scale=args.serve_scale, moderate_rate=args.serve_rate)
     if not args.no_cpu_baseline:

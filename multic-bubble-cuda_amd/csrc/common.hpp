@@ -120,9 +120,13 @@ struct Field {
 };
 
 // Population plane stride (cells) of an n-cell lattice: n rounded up to 256
-// cells, so every plane starts 1 KiB-aligned, plus MCB_PLANE_SKEW cells.
+// cells, so every plane starts 1 KiB-aligned, plus one more 256-cell block.
+// Without it the planes of a power-of-two lattice sit a power of two apart
+// (2^25 bytes for 16-bit planes at 256^3), and the 19 concurrent plane
+// streams of a collide pass contend for the same HBM channels: the skew
+// measured 0.422 -> 0.407 ms per 16-bit step at 256^3 (FP32 unchanged).
 #ifndef MCB_PLANE_SKEW
-#define MCB_PLANE_SKEW 0
+#define MCB_PLANE_SKEW 256
 #endif
 inline long long plane_stride_cells(long long n) { return (n + 255) / 256 * 256 + MCB_PLANE_SKEW; }
 

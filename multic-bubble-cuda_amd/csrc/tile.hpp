@@ -920,6 +920,17 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
             } else {
                 const FluidCell fc = fluid_cell(m, Fx, Fy, Fz);
                 const PhaseCell pc = phase_cell(c.phys, phic, m.ux, m.uy, m.uz, ncx, ncy, ncz);
+                // BGK phase collision (tau_g != 1): the 19 pre-collision g
+                // (post-stream, gathered) are loaded together here, so their
+                // latency overlaps the fluid collision instead of stalling each
+                // g* store in turn.
+                float gpre[Q];
+                if constexpr (MODE == COLLIDE_STEP_BGK) {
+                    for_links([&](auto lc) {
+                        constexpr int l = decltype(lc)::value;
+                        gpre[l] = gsrc.template ldb<l>(col.template src<l>(), zs[1 - CZ[l]] * 4u);
+                    });
+                }
                 for_links([&](auto lc) {
                     constexpr int l = decltype(lc)::value;
                     fdst.template stb<l>(col.me(), z1 * 4u, fluid_post<l>(fc, c.phys, d));
@@ -940,8 +951,7 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                     if constexpr (MODE == COLLIDE_STEP) {
                         gdst.template stb<l>(col.me(), z1 * 4u, phase_post<l>(pc, c.phys));
                     } else {
-                        const float gpre = gsrc.template ldb<l>(col.template src<l>(), zs[1 - CZ[l]] * 4u);
-                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post_bgk<l>(pc, c.phys, d, gpre));
+                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post_bgk<l>(pc, c.phys, d, gpre[l]));
                     }
                 });
             }

@@ -75,7 +75,9 @@ def test_serve_run_rejects_bad_arguments(mcb):
 
 
 def test_case_bytes(mcb):
-    n = 64 ** 3
+    # Every plane and field of a case is n cells rounded up to 256, plus one
+    # 256-cell block of skew between planes.
+    n = 64 ** 3 + 256
     # tau_g != 1: f and g populations (A/B); 16 bits add a second phi (g anchor)
     p = mcb.make_params(64, 64, 64, tau_g=0.8)
     assert mcb.case_bytes(p) == 4 * 19 * n * 4 + n * 4
