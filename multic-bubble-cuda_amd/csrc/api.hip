@@ -73,6 +73,17 @@ struct mcb_ctx {
     // into the face populations of f[ref_reset_f], which the next step reads
     // once; after it they are reset to w_l (-1: nothing pending).
     int ref_reset_f = -1;
+    // Fused path: two steps captured in a hipGraph per A/B parity, with the
+    // launch arguments they were captured with; a graph is replayed only
+    // while the next two steps would launch exactly those (same views, tile
+    // and collide mode), and recaptured otherwise.
+    struct FusedGraph {
+        hipGraphExec_t exec = nullptr;
+        CaseView v[2];
+        Tiling t{};
+        int mode = -1;
+    } fgraph[2];
+    bool graphs_ok = true;
 };
 
 struct mcb_batch {
@@ -244,6 +255,74 @@ mcb_status fused_step(mcb_ctx* c) {
     return MCB_OK;
 }
 
+// The views of the next two fused steps, without advancing the context.
+void next_two_views(mcb_ctx* c, CaseView (&v)[2]) {
+    const int cur = c->cur, gm = c->gm;
+    float* const phi = c->phi;
+    v[0] = fused_view(c);
+    c->cur ^= 1;
+    if (c->gm != GM_POPULATIONS) {
+        c->gm = GM_MOMENTS;
+        c->phi = c->phis[c->cur ^ 1];
+    }
+    v[1] = fused_view(c);
+    c->cur = cur;
+    c->gm = gm;
+    c->phi = phi;
+}
+
+void drop_fused_graphs(mcb_ctx* c) {
+    for (auto& g : c->fgraph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+}
+
+// Two fused steps from a hipGraph (captured on first use for this parity and
+// these launch arguments); *done is false, and nothing is launched, when the
+// runtime cannot capture, after which the context steps eagerly.
+mcb_status fused_two_steps_graph(mcb_ctx* c, bool* done) {
+    *done = false;
+    if (!c->graphs_ok) return MCB_OK;
+    CaseView v[2];
+    next_two_views(c, v);
+    auto& g = c->fgraph[c->cur];
+    const int mode = step_mode(c);
+    const bool same = g.exec && g.mode == mode && g.t.tx == c->tiling.tx && g.t.ty == c->tiling.ty &&
+                      g.t.zc == c->tiling.zc && std::memcmp(g.v, v, sizeof(v)) == 0;
+    if (!same) {
+        if (g.exec) MCB_HIP(c, hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            c->graphs_ok = false;
+            return MCB_OK;
+        }
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = launch_phase(v[k], c->tiling, c->stream);
+            if (e == hipSuccess) e = launch_collide(v[k], mode, c->tiling, c->stream);
+        }
+        const hipError_t ee = hipStreamEndCapture(c->stream, &gr);
+        if (e == hipSuccess && ee == hipSuccess) e = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (e != hipSuccess || ee != hipSuccess) {
+            (void)hipGetLastError();
+            g.exec = nullptr;
+            c->graphs_ok = false;
+            return MCB_OK;
+        }
+        std::memcpy(g.v, v, sizeof(v));
+        g.t = c->tiling;
+        g.mode = mode;
+    }
+    MCB_HIP(c, hipGraphLaunch(g.exec, c->stream));
+    for (int k = 0; k < 2; ++k) MCB_HIP(c, after_fused_step(c));
+    *done = true;
+    return MCB_OK;
+}
+
 #define MCB_OP(c, call)                                                   \
     do {                                                                  \
         mcb_status s_ = (call);                                           \
@@ -297,10 +376,24 @@ mcb_status check_ready(mcb_ctx* c) {
 
 mcb_status any_step(mcb_ctx* c, int nsteps) {
     if (c->batch) return batch_status(c, c->batch->step(nsteps));
-    for (int t = 0; t < nsteps; ++t) {
+    for (int t = 0; t < nsteps;) {
+        // Fused path: pairs of steps in the steady state (g in moment form
+        // past its initial state, or as populations; no face reset pending)
+        // replay a captured graph.
+        if (c->path == MCB_PATH_FUSED && nsteps - t >= 2 && c->gm != GM_INITIAL && c->ref_reset_f < 0) {
+            bool done = false;
+            mcb_status st = fused_two_steps_graph(c, &done);
+            if (st != MCB_OK) return st;
+            if (done) {
+                t += 2;
+                c->steps += 2;
+                continue;
+            }
+        }
         mcb_status st = c->path == MCB_PATH_FUSED ? fused_step(c) : staged_step(c);
         if (st != MCB_OK) return st;
         ++c->steps;
+        ++t;
     }
     return MCB_OK;
 }
@@ -462,7 +555,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
             }
             if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MCB_ERR_HIP);
         }
-        c->tiling = choose_tiling(*p);
+        c->tiling = choose_tiling_single(*p);
     } else {
         // The reference layout: planes of exactly n cells (src/header.cuh:21).
         c->geo.plane = n;
@@ -489,6 +582,7 @@ mcb_status mcb_destroy(mcb_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->batch.reset();
+    drop_fused_graphs(c);
     for (void* q : c->allocs) (void)hipFree(q);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);

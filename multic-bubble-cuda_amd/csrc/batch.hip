@@ -121,6 +121,24 @@ Tiling choose_tiling(const mcb_params& p) {
     return t;
 }
 
+Tiling choose_tiling_single(const mcb_params& p) {
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(std::make_tuple(p.nx, p.ny, p.nz, p.storage));
+        if (it != g_tuned.end()) return it->second;
+    }
+    // A lattice alone on the device: shorter z-chunks until the pass has two
+    // workgroups per CU (256 CUs), down to 4 planes. The reference preset
+    // (64x64x256) gets 8 tiles per z layer at 64x8, i.e. 128 workgroups with
+    // 16-plane chunks, which leaves half the CUs idle.
+    Tiling t = choose_tiling(p);
+    auto tiles = [&](const Tiling& q) {
+        return (long long)((p.nx + q.tx - 1) / q.tx) * ((p.ny + q.ty - 1) / q.ty) * ((p.nz + q.zc - 1) / q.zc);
+    };
+    while (t.zc > 4 && tiles(t) < 512) t.zc /= 2;
+    return t;
+}
+
 // ------------------------------------------------------------------ Batch
 
 Batch::Batch(int device, hipStream_t stream, bool use_graphs)

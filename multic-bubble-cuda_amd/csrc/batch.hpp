@@ -170,6 +170,9 @@ private:
 // Tile shape for a lattice: the autotuned choice for (nx, ny, nz, storage)
 // when one exists, else a size rule.
 Tiling choose_tiling(const mcb_params& p);
+// The tiling of a single-case context: as choose_tiling, with z-chunks short
+// enough that a small lattice alone still fills the device.
+Tiling choose_tiling_single(const mcb_params& p);
 // Measure every supported tile shape and z-chunk on a scratch case and
 // remember the fastest for (nx, ny, nz, storage).
 mcb_status autotune(const mcb_params& p, int device, Tiling* best, double* ms_per_step);

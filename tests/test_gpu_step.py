@@ -244,3 +244,36 @@ def test_reference_mode_fused_write_f_and_resume(gpu_lib):
     with pytest.raises(mcb.McbError):
         lat.read("g")           # the reference layout of g is kept by the staged path only
     lat.close()
+
+
+@pytest.mark.parametrize("kind", ["moments", "populations", "reference"])
+def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
+    """mcb_step replays pairs of fused steps from a captured hipGraph. Each
+    replay, with eager work (observables, a field readout) and host
+    synchronisation between replays, must give the bits of the same steps
+    launched one at a time."""
+    mcb = gpu_lib
+    kw = dict(tau_f=0.6, sigma=0.08, path=mcb.PATH_FUSED)
+    if kind == "populations":
+        kw["tau_g"] = 0.8
+    if kind == "reference":
+        kw.update(bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL)
+    p = mcb.make_params(30, 26, 22, **kw)
+    bubbles = [(15.2, 13.1, 11.3, 6.0)]
+    g, e = mcb.Lattice(p), mcb.Lattice(p)
+    g.init_bubbles(bubbles)
+    e.init_bubbles(bubbles)
+    g.step(1)                      # the first step (initial state) is eager on both
+    e.step(1)
+    for replay in range(6):
+        g.step(2)                  # one graph replay
+        e.step(1)                  # eager launches
+        e.step(1)
+        a, b = g.read("phi"), e.read("phi")
+        assert np.array_equal(a, b), f"{kind}, replay {replay}: phi differs in {int(np.count_nonzero(a != b))} cells"
+        if replay % 2 == 0:
+            ga, eb = g.observables(), e.observables()
+            for k in OBS:
+                assert np.array_equal(ga[k], eb[k]), f"{kind}, replay {replay}: {k}"
+    g.close()
+    e.close()
