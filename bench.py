@@ -151,8 +151,9 @@ def storage_form(mcb, case, storage, steps, warmup):
 
 def batching(mcb, n_cases=64, steps=2000, seed=1000):
     """The uneven batch: n_cases FP32 cases (sides 32..128) advanced together
-    (one launch per pass per tile shape, graph-replayed) vs the same cases
-    advanced one at a time on single-case contexts."""
+    (one tile shape for all, so one phase and one collide launch per step,
+    graph-replayed) vs the same cases advanced one at a time on single-case
+    contexts with each side's autotuned tile."""
     cases = [mcb.case_preset("batch", seed + i) for i in range(n_cases)]
     sides = sorted({c.params.nx for c in cases})
     tuned = {}

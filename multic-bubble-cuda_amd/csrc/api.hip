@@ -284,6 +284,11 @@ void drop_fused_graphs(mcb_ctx* c) {
 mcb_status fused_two_steps_graph(mcb_ctx* c, bool* done) {
     *done = false;
     if (!c->graphs_ok) return MCB_OK;
+    // A caller capturing the context's stream into a graph of its own gets
+    // the eager launches recorded there.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    MCB_HIP(c, hipStreamIsCapturing(c->stream, &cap));
+    if (cap != hipStreamCaptureStatusNone) return MCB_OK;
     CaseView v[2];
     next_two_views(c, v);
     auto& g = c->fgraph[c->cur];

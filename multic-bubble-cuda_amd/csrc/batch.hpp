@@ -173,6 +173,9 @@ Tiling choose_tiling(const mcb_params& p);
 // The tiling of a single-case context: as choose_tiling, with z-chunks short
 // enough that a small lattice alone still fills the device.
 Tiling choose_tiling_single(const mcb_params& p);
+// The tiling of a batch case: one shape for all FP32 cases (one launch per
+// pass), the 16-bit rule of choose_tiling otherwise.
+Tiling batch_tiling(const mcb_params& p);
 // Measure every supported tile shape and z-chunk on a scratch case and
 // remember the fastest for (nx, ny, nz, storage).
 mcb_status autotune(const mcb_params& p, int device, Tiling* best, double* ms_per_step);

@@ -277,3 +277,39 @@ def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
                 assert np.array_equal(ga[k], eb[k]), f"{kind}, replay {replay}: {k}"
     g.close()
     e.close()
+
+
+def test_steps_inside_a_caller_graph(gpu_lib):
+    """A caller capturing the context's stream into its own hipGraph records
+    mcb_step's eager launches (the library captures nothing of its own while
+    the stream is capturing); replaying the caller's graph advances the
+    lattice as the same steps launched directly."""
+    import ctypes
+    mcb = gpu_lib
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    stream, graph, exe = vp(), vp(), vp()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    p = mcb.make_params(30, 26, 22, tau_f=0.6, sigma=0.08, path=mcb.PATH_FUSED)
+    bubbles = [(15.2, 13.1, 11.3, 6.0)]
+    g = mcb.Lattice(p, stream=stream)
+    e = mcb.Lattice(p)
+    g.init_bubbles(bubbles)
+    e.init_bubbles(bubbles)
+    g.step(1)
+    e.step(1)
+    assert hip.hipStreamBeginCapture(stream, 1) == 0          # thread-local capture mode
+    g.step(2)                                                 # recorded, not run
+    assert hip.hipStreamEndCapture(stream, ctypes.byref(graph)) == 0
+    assert hip.hipGraphInstantiate(ctypes.byref(exe), graph, None, None, ctypes.c_size_t(0)) == 0
+    for replay in range(3):
+        assert hip.hipGraphLaunch(exe, stream) == 0
+        assert hip.hipStreamSynchronize(stream) == 0
+        e.step(2)
+        a, b = g.read("phi"), e.read("phi")
+        assert np.array_equal(a, b), f"replay {replay}: phi differs in {int(np.count_nonzero(a != b))} cells"
+    hip.hipGraphExecDestroy(exe)
+    hip.hipGraphDestroy(graph)
+    g.close()
+    e.close()
+    hip.hipStreamDestroy(stream)
