@@ -34,9 +34,10 @@ B_ALG_STEP = 2 * 19 * 2 * 4           # f and g, 19 pops, read + write, FP32: 30
 B_ALG_COLLIDE = 4 * 19 * 4 + 4        # k_collide: f read + write, g read + write, phi read: 308 B per cell
 B_ALG_PHASE = 19 * 4 + 4              # k_phase: g read, phi write: 80 B per cell
 # g in moment form (tau_g = 1, the benchmark case): g*_l(y) is recomputed from
-# phi, u, n of its source cell instead of being stored.
-B_ALG_COLLIDE_GM = 2 * 19 * 4 + 6 * 4 + 4   # f read, f write, u and n write, phi read: 180 B per cell
-B_ALG_PHASE_GM = 7 * 4 + 4                  # phi, u, n read; phi write: 32 B per cell
+# phi and u of its source cell and its normal n, itself recomputed from phi,
+# instead of being stored.
+B_ALG_COLLIDE_GM = 2 * 19 * 4 + 3 * 4 + 4   # f read, f write, u write, phi read: 168 B per cell
+B_ALG_PHASE_GM = 4 * 4 + 4                  # phi, u read; phi write: 20 B per cell
 METRIC = "MLUPS at 256³ + % of HBM roofline; serving p50/p99 latency vs plain solves"
 
 
@@ -340,7 +341,7 @@ def main():
             "b_alg_per_lu": B_ALG_STEP,
             "achieved_gbs": round(mlups * 1e6 * B_ALG_STEP / 1e9, 1),
             "frac": round(mlups * 1e6 * B_ALG_STEP / 1e9 / HBM_PEAK_GBS, 4),
-            "g_storage": "moments (phi, u, n)" if moment_form else "populations",
+            "g_storage": "moments (phi, u; n recomputed)" if moment_form else "populations",
             "b_moved_per_lu": b_collide + b_phase,
             "moved_frac": round(mlups * 1e6 * (b_collide + b_phase) / 1e9 / HBM_PEAK_GBS, 4),
             "k_phase_ms": round(avg_phase_ms, 5),

@@ -62,6 +62,16 @@ typedef enum mcb_storage { MCB_STORE_FP32 = 0, MCB_STORE_FP16 = 1, MCB_STORE_BF1
  * tau_g = 1 without moving_wall (the reference preset); STAGED otherwise. */
 typedef enum mcb_path { MCB_PATH_AUTO = 0, MCB_PATH_FUSED = 1, MCB_PATH_STAGED = 2 } mcb_path;
 
+/* Interface normal n between steps on the FUSED FP32 path at tau_g = 1 (g in
+ * moment form: the phase pass rebuilds g from phi, u and n of each source
+ * cell). STORED: the collide pass writes n beside u. RECOMPUTED: the phase
+ * pass recomputes n from phi with gradCalc's operations (src/lbm.cu:50-71),
+ * the same bits, 24 fewer bytes per cell and step and more arithmetic. AUTO:
+ * RECOMPUTED where the phase pass runs z-chunks of 32 planes or more (large
+ * lattices), STORED otherwise. Results are identical in every mode; other
+ * paths and forms ignore it. */
+typedef enum mcb_normals { MCB_NORMALS_AUTO = 0, MCB_NORMALS_STORED = 1, MCB_NORMALS_RECOMPUTED = 2 } mcb_normals;
+
 typedef struct mcb_params {
     int nx, ny, nz;     /* lattice sides (>= 3; reference 64 x 64 x 256, src/var.cu:3-4) */
     int bc_mode;        /* mcb_bc_mode */
@@ -74,7 +84,8 @@ typedef struct mcb_params {
     float sigma;        /* surface tension H_SIGMA (src/var.cu:25) */
     float sharp_c;      /* interface sharpening H_SHARP_C; < 0 selects 0.15f*3.0f (src/var.cu:24) */
     int moving_wall;    /* 1: also run the reference's movingWallBC (dead code, src/lbm.cu:430-461) */
-    int reserved[7];
+    int normals;        /* mcb_normals */
+    int reserved[6];
 } mcb_params;
 
 typedef struct mcb_bubble {

@@ -67,7 +67,7 @@ struct mcb_ctx {
     int gm = GM_POPULATIONS;
     float* phis[2] = {nullptr, nullptr};
     float* gmu[3] = {nullptr, nullptr, nullptr};
-    float* gmn[3] = {nullptr, nullptr, nullptr};
+    float* gmn[3] = {nullptr, nullptr, nullptr};   // null: the phase pass recomputes n
     long long steps = 0;   // steps since the initial state (non-batch paths)
     // Fused path with REFERENCE boundaries: a written F put arbitrary values
     // into the face populations of f[ref_reset_f], which the next step reads
@@ -155,13 +155,22 @@ int resolve_path(const mcb_params& p) {
                                                                                     : MCB_PATH_STAGED;
 }
 
+// Whether a single-case FP32 moment-form context recomputes n in the phase
+// pass: as asked, or (AUTO) where its tiling gives the pass long z-chunks.
+bool ctx_recomputes_n(const mcb_params& p) {
+    if (p.normals != MCB_NORMALS_AUTO) return p.normals == MCB_NORMALS_RECOMPUTED;
+    const Geo g{p.nx, p.ny, p.nz, p.nx * p.ny, (long long)p.nx * p.ny * p.nz, 0};
+    return phase_recomputes_n(g, choose_tiling_single(p));
+}
+
 size_t footprint(const mcb_params& p, int path) {
     // 16-bit: the case plus the batch's ten macro fields.
     if (p.storage != MCB_STORE_FP32) return case_bytes(p) + 10 * (size_t)plane_stride((long long)p.nx * p.ny * p.nz) * 4;
     const long long n = (long long)p.nx * p.ny * p.nz;
     const long long P = plane_stride(n);
     size_t floats = 0;
-    if (path == MCB_PATH_FUSED && p.tau_g == 1.0f) floats = 2 * 19 * (size_t)P + 18 * (size_t)P;   // g in moment form
+    if (path == MCB_PATH_FUSED && p.tau_g == 1.0f)   // g in moment form: phi twice, u, and n unless recomputed
+        floats = 2 * 19 * (size_t)P + (ctx_recomputes_n(p) ? 15 : 18) * (size_t)P;
     else if (path == MCB_PATH_FUSED) floats = 4 * 19 * (size_t)P + 11 * (size_t)P;
     else floats = 3 * 19 * (size_t)P + 19 * (size_t)P;
     return floats * sizeof(float) + (2 * kReduceBlocks + 2) * sizeof(double);
@@ -407,7 +416,8 @@ mcb_status any_step(mcb_ctx* c, int nsteps) {
 // reference variable (src/var.cu:13-18): with ghost-layer boundaries the
 // kernels leave the face values of phi and the macroscopic fields from one
 // iteration to the next, so they are state too; fused FP32, f of the current
-// buffer and g as populations or in moment form (the state's phi, u, n).
+// buffer and g as populations or in moment form (the state's phi and u; n
+// is gradCalc of that phi, rebuilt on loading where the context stores it).
 std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     if (c->path == MCB_PATH_STAGED) {
@@ -425,10 +435,8 @@ std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
         parts.push_back({"g", c->g[c->cur], 19, N, P, 4, false});
     } else {
         static const char* const un[3] = {"u0", "u1", "u2"};
-        static const char* const nn[3] = {"n0", "n1", "n2"};
         parts.push_back({"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false});
         for (int k = 0; k < 3; ++k) parts.push_back({un[k], c->gmu[k], 1, N, N, 4, false});
-        for (int k = 0; k < 3; ++k) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
     }
     return parts;
 }
@@ -529,7 +537,7 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
     } else if (path == MCB_PATH_FUSED) {
         c->geo.plane = P;
         // At tau_g = 1 g is kept in moment form: phi (two buffers), u and n
-        // instead of two sets of 19 planes.
+        // (or n recomputed by the phase pass) instead of two sets of 19 planes.
         const bool moments = p->tau_g == 1.0f;
         for (int b = 0; b < 2; ++b) {
             if ((st = alloc(c, &c->f[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
@@ -542,9 +550,13 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
         if (moments) {
             c->gm = GM_INITIAL;
             c->phis[0] = c->phi;
-            float** gmf[] = {&c->phis[1], &c->gmu[0], &c->gmu[1], &c->gmu[2], &c->gmn[0], &c->gmn[1], &c->gmn[2]};
+            float** gmf[] = {&c->phis[1], &c->gmu[0], &c->gmu[1], &c->gmu[2]};
             for (auto* q : gmf)
                 if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
+            // n is stored beside u unless the phase pass recomputes it.
+            if (!ctx_recomputes_n(*p))
+                for (int k = 0; k < 3; ++k)
+                    if ((st = alloc(c, &c->gmn[k], (size_t)P)) != MCB_OK) return bail(st);
         }
         if (p->bc_mode == MCB_BC_REFERENCE) {
             // Face cells never collide: their f*_l stay w_l in both buffers
@@ -553,11 +565,10 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
             for (int b = 0; b < 2; ++b)
                 if (launch_init_pull(c->phi, c->f[b], nullptr, c->geo, c->phys, c->stream) != hipSuccess)
                     return bail(MCB_ERR_HIP);
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < 3; ++k)
                 if (hipMemsetAsync(c->gmu[k], 0, (size_t)P * sizeof(float), c->stream) != hipSuccess ||
-                    hipMemsetAsync(c->gmn[k], 0, (size_t)P * sizeof(float), c->stream) != hipSuccess)
+                    (c->gmn[k] && hipMemsetAsync(c->gmn[k], 0, (size_t)P * sizeof(float), c->stream) != hipSuccess))
                     return bail(MCB_ERR_HIP);
-            }
             if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MCB_ERR_HIP);
         }
         c->tiling = choose_tiling_single(*p);
@@ -1148,7 +1159,13 @@ mcb_status mcb_load_state(mcb_ctx* c, const void* buf, size_t bytes) {
         c->ref_reset_f = -1;   // the image's face populations are w_l
     }
     if ((st = state_image_load(c->stream, buf, bytes, ctx_state_parts(c), &c->err)) != MCB_OK) return st;
-    if (c->path == MCB_PATH_FUSED && c->gm != GM_POPULATIONS) c->phi = c->phis[c->cur ^ 1];
+    if (c->path == MCB_PATH_FUSED && c->gm != GM_POPULATIONS) {
+        c->phi = c->phis[c->cur ^ 1];
+        if (c->gmn[0]) {   // n is not in the image: gradCalc of the state's phi
+            MCB_HIP(c, launch_normals(fused_view(c), c->stream));
+            MCB_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
     c->steps = h.steps;
     return MCB_OK;
 }

@@ -310,8 +310,10 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
         c->phi[1] = static_cast<float*>(q);
     }
     if (moments) {
-        // FP32: u and n (collisionPhase's inputs, bit-exact); 16 bits: the
-        // vector B = 3 phi u + phi_n n in FP16 (phase_tile_ab).
+        // FP32: u and n (collisionPhase's inputs, bit-exact; a batch's
+        // 8-plane chunks are too short for the phase pass to recompute n
+        // profitably); 16 bits: the vector B = 3 phi u + phi_n n in FP16
+        // (phase_tile_ab).
         const bool fp32 = p.storage == MCB_STORE_FP32;
         for (int k = 0; k < (fp32 ? 6 : 3); ++k) {
             if ((st = alloc((size_t)P * (fp32 ? sizeof(float) : 2), &q)) != MCB_OK) return st;
@@ -669,9 +671,10 @@ mcb_status Batch::to_populations(CaseState* c) {
 // --------------------------------------------------------------- state
 
 // What the next step of a case reads (view(cur)): f of the current buffer;
-// g as populations, or its moment form (u, n in FP32; B in 16 bits); the
+// g as populations, or its moment form (u in FP32; B in 16 bits); the
 // state's phi (the moment form's phi, the 16-bit g anchor); the 16-bit mass
-// target and the correction pending for the next phase pass.
+// target and the correction pending for the next phase pass. n is not part
+// of an image: it is gradCalc of the state's phi, rebuilt on loading.
 std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     const unsigned eb = pop_bytes(c->storage);
@@ -681,11 +684,8 @@ std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
     if (c->gm != GM_POPULATIONS) {
         static const char* const un[3] = {"u0", "u1", "u2"};
-        static const char* const nn[3] = {"n0", "n1", "n2"};
         for (int k = 0; k < 3; ++k)
             if (c->gmu[k]) parts.push_back({un[k], c->gmu[k], 1, N, N, c->storage == MCB_STORE_FP32 ? 4u : 2u, false});
-        for (int k = 0; k < 3; ++k)
-            if (c->gmn[k]) parts.push_back({nn[k], c->gmn[k], 1, N, N, 4, false});
     }
     if (c->mass) parts.push_back({"mass", mass, 1, 2, 2, 8, true});
     if (c->massq) parts.push_back({"massq", mass + 2, 1, 3, 3, 8, true});
@@ -755,6 +755,10 @@ mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
         std::memcpy(&m.target, &m5[2], sizeof(double));
         m.delta[c->cur] = m5[4];
         B_HIP(hipMemcpy(c->massq, &m, sizeof(m), hipMemcpyHostToDevice));
+    }
+    if (c->gm != GM_POPULATIONS && c->gmn[0]) {
+        B_HIP(launch_normals(c->view(c->cur), stream_));
+        B_HIP(hipStreamSynchronize(stream_));
     }
     c->steps = h.steps;
     c->phi_last = c->phi[c->cur ^ 1];

@@ -37,7 +37,7 @@ struct CaseState {
     float* phi_last = nullptr;      // written by the most recent phase pass
     int gm = GM_POPULATIONS;        // FP32 at tau_g = 1: g in moment form (GMode)
     float* gmu[3] = {nullptr, nullptr, nullptr};
-    float* gmn[3] = {nullptr, nullptr, nullptr};
+    float* gmn[3] = {nullptr, nullptr, nullptr};   // FP32 moment form: n, stored by the collide pass
     MassCell* mass = nullptr;       // 16-bit storage: mass-drift correction state
     MassQ* massq = nullptr;         // 16-bit storage in moment form: its correction state
     double* partial = nullptr;      // one FP64 partial per phase work item

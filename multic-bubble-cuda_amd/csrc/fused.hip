@@ -51,9 +51,14 @@ __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, un
     st32(c.phi, me, v);
 }
 
-// Moment-form phase pass (phase_tile_gm): the collide pass's tiles and grid.
-template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_phase_gm(CaseView c) {
+// Moment-form phase pass (phase_tile_gm, or phase_tile_gm_rn when n is
+// recomputed): the collide pass's tiles and grid. The recomputing form at
+// 512-thread tiles is held to 80 VGPRs: three workgroups per CU (LDS ~49 KB
+// each).
+template <int TX, int TY, bool RN> constexpr int phase_gm_waves_per_eu() { return RN && TX * TY == 512 ? 6 : 1; }
+template <int TX, int TY, bool RN>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(phase_gm_waves_per_eu<TX, TY, RN>())))
+void k_phase_gm(CaseView c) {
     const Geo& G = c.geo;
     const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
     unsigned t = xcd_order(blockIdx.x, gridDim.x);
@@ -61,7 +66,24 @@ __global__ __launch_bounds__(TX * TY) void k_phase_gm(CaseView c) {
     t /= gx;
     const int y0 = (int)(t % gy) * TY;
     const int zb = (int)(t / gy) * c.zc;
-    phase_tile_gm<TX, TY>(c, x0, y0, zb, min(zb + c.zc, G.nz));
+    if constexpr (RN) phase_tile_gm_rn<TX, TY>(c, x0, y0, zb, min(zb + c.zc, G.nz));
+    else phase_tile_gm<TX, TY>(c, x0, y0, zb, min(zb + c.zc, G.nz));
+}
+
+// n of every cell from the state phi (normal_at), into gm_n: a stored-normal
+// case resumed from a state image, which keeps phi and u only.
+__global__ __launch_bounds__(PBX * PBY) void k_normals(CaseView c) {
+    const Geo& G = c.geo;
+    const int x = blockIdx.x * PBX + threadIdx.x;
+    const int y = blockIdx.y * PBY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= G.nx || y >= G.ny) return;
+    float nx, ny, nz;
+    normal_at(c, x, y, z, nx, ny, nz);
+    const unsigned me = (unsigned)(z * G.nxy + y * G.nx + x);
+    st32(c.gm_n[0], me, nx);
+    st32(c.gm_n[1], me, ny);
+    st32(c.gm_n[2], me, nz);
 }
 
 // --------------------------------------------------------------- k_collide
@@ -100,8 +122,8 @@ __global__ __launch_bounds__(PBX * PBY) void k_init_pull(const float* __restrict
 }
 
 // Moment form -> the reference's post-stream g layout:
-// dst_l(x) = g*_l(x - c_l) recomputed from phi, u, n (or w_l phi0(x) before
-// the first step).
+// dst_l(x) = g*_l(x - c_l) recomputed from phi and u, with n recomputed from
+// phi (or w_l phi0(x) before the first step).
 __global__ __launch_bounds__(PBX * PBY) void k_gm_to_ref(CaseView c, float* __restrict__ dst) {
     const Geo& G = c.geo;
     const int x = blockIdx.x * PBX + threadIdx.x;
@@ -118,7 +140,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_gm_to_ref(CaseView c, float* __re
         constexpr int l = decltype(lc)::value;
         float v;
         if (c.gm == GM_INITIAL) v = c.phys.w[WC[l]] * ld32(c.phi_prev, me);
-        else v = g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
+        else v = g_from_moments<l>(c, x - CX[l], y - CY[l], z - CZ[l]);
         dst[(size_t)l * G.plane + me] = v;
     });
 }
@@ -297,22 +319,27 @@ bool tiling_supported(const Tiling& t) {
 int tiling_count() { return (int)(sizeof(kTilings) / sizeof(kTilings[0])); }
 Tiling tiling_at(int i) { return kTilings[i]; }
 
+int phase_chunk(const Geo& geo, const Tiling& t) {
+    const unsigned layer = (unsigned)((geo.nx + t.tx - 1) / t.tx) * (unsigned)((geo.ny + t.ty - 1) / t.ty);
+    int zc = std::min(4 * t.zc, geo.nz);
+    while (zc > t.zc && layer * (unsigned)((geo.nz + zc - 1) / zc) < 256u) zc = (zc + 1) / 2;
+    return zc;
+}
+
+bool phase_recomputes_n(const Geo& geo, const Tiling& t) { return phase_chunk(geo, t) >= 32; }
+
 hipError_t launch_phase(const CaseView& c0, const Tiling& t, hipStream_t s) {
     if (c0.gm == GM_MOMENTS) {
         CaseView c = c0;
-        // Longer z-chunks than the collide pass: a chunk re-reads one plane of
-        // (phi, u, n) below and one above, so 16-plane chunks fetched 18% more
-        // than the algorithmic bytes at 256^3 and 64-plane chunks 4%, in the
-        // same time; halved while the pass would have fewer workgroups than
-        // CUs (small lattices).
-        const unsigned layer = (unsigned)((c.geo.nx + t.tx - 1) / t.tx) * (unsigned)((c.geo.ny + t.ty - 1) / t.ty);
-        c.zc = std::min(4 * t.zc, c.geo.nz);
-        while (c.zc > t.zc && layer * (unsigned)((c.geo.nz + c.zc - 1) / c.zc) < 256u) c.zc = (c.zc + 1) / 2;
-        const unsigned tiles = layer * (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
-#define MCB_PHASE_GM(TX, TY)                                                                        \
-    if (t.tx == TX && t.ty == TY) {                                                                 \
-        hipLaunchKernelGGL((k_phase_gm<TX, TY>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);            \
-        return hipGetLastError();                                                                   \
+        c.zc = phase_chunk(c.geo, t);
+        const unsigned tiles = (unsigned)((c.geo.nx + t.tx - 1) / t.tx) * (unsigned)((c.geo.ny + t.ty - 1) / t.ty) *
+                               (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
+        const bool rn = c.gm_n[0] == nullptr;
+#define MCB_PHASE_GM(TX, TY)                                                                                     \
+    if (t.tx == TX && t.ty == TY) {                                                                              \
+        if (rn) hipLaunchKernelGGL((k_phase_gm<TX, TY, true>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);          \
+        else hipLaunchKernelGGL((k_phase_gm<TX, TY, false>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);            \
+        return hipGetLastError();                                                                                \
     }
         MCB_TILES_FP32(MCB_PHASE_GM)
 #undef MCB_PHASE_GM
@@ -348,6 +375,12 @@ hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, uns
 
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s) {
     hipLaunchKernelGGL(k_init_pull, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(const CaseView& c, hipStream_t s) {
+    if (!c.gm_n[0]) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_normals, cell_grid(c.geo), dim3(PBX, PBY, 1), 0, s, c);
     return hipGetLastError();
 }
 

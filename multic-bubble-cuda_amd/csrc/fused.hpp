@@ -22,7 +22,8 @@ enum CollideMode { COLLIDE_STEP = 0, COLLIDE_STEP_BGK = 1, COLLIDE_MACROS = 2 };
 
 // How a case's state holds g (CaseView::gm).
 //   GM_POPULATIONS : 19 planes of g* (pull form), any tau_g.
-//   GM_MOMENTS     : phi, u, n of the last collision (tau_g = 1 only).
+//   GM_MOMENTS     : phi, u and n of the last collision (tau_g = 1 only); n
+//                    is stored or recomputed from phi (CaseView::gm_n).
 //   GM_INITIAL     : the initial state g_l(x) = w_l phi0(x) (initDist), before
 //                    the first step of the moment form; phi_prev holds phi0.
 enum GMode { GM_POPULATIONS = 0, GM_MOMENTS = 1, GM_INITIAL = 2 };
@@ -82,16 +83,20 @@ struct CaseView {
     const float* phi_prev;  // 16-bit g populations: phi of the previous step (anchor of g);
                             // moment form: phi of the previous step's collision
     // Moment form of g (FP32, tau_g = 1): g*_l(y) is collisionPhase of
-    // (phi, u, n) at y, so the state keeps u and n (gm_u, gm_n) instead of
-    // 19 g planes and the phase pass recomputes g*_l(x - c_l). 16-bit forms:
-    // gm_u holds B = 3 phi u + phi_n n as FP16 (2 bytes per cell; ld_b).
+    // (phi, u, n) at y, so the state keeps u (gm_u) instead of 19 g planes
+    // and the phase pass recomputes g*_l(x - c_l). n is gradCalc of phi
+    // around y: with gm_n set the collide pass stores it, with gm_n null the
+    // phase pass recomputes it (phase_tile_gm_rn; 24 fewer bytes per cell
+    // and step, more arithmetic per plane, so it pays on long z-chunks).
+    // 16-bit forms: gm_u holds B = 3 phi u + phi_n n as FP16 (2 bytes per
+    // cell; ld_b), gm_n is null.
     float* gm_u[3];
     float* gm_n[3];
     int gm;                 // GM_POPULATIONS, GM_MOMENTS or GM_INITIAL
     // MCB_BC_PERIODIC, or MCB_BC_REFERENCE (FP32 moment form of a single
     // case): the reference's ghost-layer boundaries (src/lbm.cu:365-425) in
-    // pull form. Face cells never collide; their f*_l stays w_l and their u
-    // and n stay 0, so an interior cell that gathers from a face e receives
+    // pull form. Face cells never collide; their f*_l stays w_l, their u
+    // stays 0 and their n is 0 (gradCalc skips them), so an interior cell that gathers from a face e receives
     // rho(e) w_l = w_l (fgBoundary's f; face rho stays 1) and w_l phi(e)
     // (fgBoundary's g). The phase pass writes the state phi of a face cell as
     // boundaryConditions does: the previous state phi at the clamped interior
@@ -134,12 +139,22 @@ Tiling tiling_at(int i);
 // ---- single case, FP32 populations (bit-exact form)
 // Phase pass; in moment form (gm == GM_MOMENTS) it runs on the tiles t.
 hipError_t launch_phase(const CaseView& c, const Tiling& t, hipStream_t s);
+// z-chunk of the moment-form phase pass on tiles t: four times the collide
+// pass's (each chunk also reads the planes around it), halved while the pass
+// would have fewer than 256 workgroups.
+int phase_chunk(const Geo& geo, const Tiling& t);
+// Whether a single-case FP32 moment-form context recomputes n in the phase
+// pass (CaseView::gm_n null) rather than storing it: on chunks of 32 planes
+// or more, where the 24 bytes per cell saved outweigh the per-chunk start.
+bool phase_recomputes_n(const Geo& geo, const Tiling& t);
+// n from the state phi into gm_n (a stored-normal case loaded from an image).
+hipError_t launch_normals(const CaseView& c, hipStream_t s);
 hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStream_t s);
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
 // initTensor + initDist (src/init.cu:21-24, 69-79). g may be null (moment form).
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s);
 // Moment form of g -> reference layout (post-stream g_l(x), 19 planes):
-// FP32 (phi, u, n) form, and the (phi, B) form of the 16-bit storage forms.
+// FP32 (phi, u) form (n recomputed), and the (phi, B) form of the 16-bit storage forms.
 hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s);
 hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->

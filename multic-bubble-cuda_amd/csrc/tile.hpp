@@ -10,7 +10,8 @@
 //               where phi(y) is the phase the source cell had when it
 //               collided (an FP32 field). At tau_g = 1 g is not stored as
 //               populations at all but in a moment form: (phi, u, n) in FP32
-//               for StoreF32 (phase_tile_gm), (phi, B) with B in FP16 for the
+//               for StoreF32 (phase_tile_gm; phase_tile_gm_rn recomputes n
+//               from phi instead), (phi, B) with B in FP16 for the
 //               16-bit forms (phase_tile_ab). The offsets are small (density, velocity
 //               and sharpening terms), so 16 bits keep about 1e-5 of w_l
 //               (FP16). K_l is a power of two near 2^12 / w_l: an
@@ -189,30 +190,61 @@ template <unsigned EB> struct Columns {
 
 // ------------------------------------------------------------------ phase
 
-// phiCalc (src/lbm.cu:18-25) for one cell of a pull-form state: the
-// post-stream g_l(x) is g*_l(x - c_l). xs/ys/zs: offsets of x-1, x, x+1 etc.
-//
-// Moment form (FP32, tau_g = 1): g*_l(y) is recomputed from phi, u, n of the
-// source cell with collisionPhase's own expressions (lbm.cu:319-326), so the
-// sum carries the same bits as with stored populations. The initial state is
-// initDist's g_l(x) = w_l phi0(x) (init.cu:69-79).
-template <int L> __device__ __forceinline__ float g_from_moments(const CaseView& c, unsigned y) {
-    const PhaseCell pc = phase_cell(c.phys, ld32(c.phi_prev, y), ld32(c.gm_u[0], y), ld32(c.gm_u[1], y),
-                                    ld32(c.gm_u[2], y), ld32(c.gm_n[0], y), ld32(c.gm_n[1], y), ld32(c.gm_n[2], y));
+// Moment form (FP32, tau_g = 1): g*_l(y) is recomputed from the source cell's
+// phi, u and normal n with collisionPhase's own expressions (lbm.cu:319-326),
+// so the phase sum carries the same bits as with stored populations. n is
+// either stored by the collide pass (CaseView::gm_n set) or recomputed from
+// phi around the source with gradCalc's expressions (lbm.cu:50-71), which
+// gives the bits the collide pass computed. The initial state is initDist's
+// g_l(x) = w_l phi0(x) (init.cu:69-79).
+
+// n at cell (x, y, z) of the state phi (c.phi_prev), read from global memory,
+// as the collide pass computed it; coordinates up to one cell outside the
+// lattice name its periodic image. A ghost-layer face keeps n = 0 (gradCalc
+// updates interior cells only).
+__device__ __forceinline__ void normal_at(const CaseView& c, int x, int y, int z, float& nx, float& ny, float& nz) {
+    const Geo& G = c.geo;
+    if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z)) {
+        nx = ny = nz = 0.0f;
+        return;
+    }
+    const Derived d(c.phys);
+    auto nb = [&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        return ld32(c.phi_prev, (unsigned)(wrapmod(z + CZ[l], G.nz) * G.nxy + wrapmod(y + CY[l], G.ny) * G.nx +
+                                           wrapmod(x + CX[l], G.nx)));
+    };
+    float ind;
+    grad_normal(nb, d, nx, ny, nz, ind);
+}
+
+// g*_l of the source cell (x, y, z) (periodic images allowed), from global
+// memory (the moment form's readout as populations).
+template <int L> __device__ __forceinline__ float g_from_moments(const CaseView& c, int x, int y, int z) {
+    const Geo& G = c.geo;
+    const unsigned o = (unsigned)(wrapmod(z, G.nz) * G.nxy + wrapmod(y, G.ny) * G.nx + wrapmod(x, G.nx));
+    float nx, ny, nz;
+    if (c.gm_n[0]) {
+        nx = ld32(c.gm_n[0], o);
+        ny = ld32(c.gm_n[1], o);
+        nz = ld32(c.gm_n[2], o);
+    } else {
+        normal_at(c, x, y, z, nx, ny, nz);
+    }
+    const PhaseCell pc = phase_cell(c.phys, ld32(c.phi_prev, o), ld32(c.gm_u[0], o), ld32(c.gm_u[1], o),
+                                    ld32(c.gm_u[2], o), nx, ny, nz);
     return phase_post<L>(pc, c.phys);
 }
 
+// phiCalc (src/lbm.cu:18-25) for one cell of a pull-form state: the
+// post-stream g_l(x) is g*_l(x - c_l). xs/ys/zs: offsets of x-1, x, x+1 etc.
+// From stored populations, or before the first step from initDist's
+// g_l = w_l phi0 (GM_INITIAL); the moment form runs phase_tile_gm (FP32) or
+// phase_tile_ab (16-bit forms) instead.
 template <class S>
 __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const unsigned (&xs)[3],
                                            const unsigned (&ys)[3], const unsigned (&zs)[3]) {
     float sum = 0.0f;
-    if (c.gm == GM_MOMENTS) {
-        for_links([&](auto lc) {
-            constexpr int l = decltype(lc)::value;
-            sum += g_from_moments<l>(c, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]);
-        });
-        return sum;
-    }
     if (c.gm == GM_INITIAL) {
         const float p0 = ld32(c.phi_prev, zs[1] + ys[1] + xs[1]);
         for_links([&](auto lc) { sum += c.phys.w[WC[decltype(lc)::value]] * p0; });
@@ -263,8 +295,54 @@ __device__ __forceinline__ float phase_link(float w, float wphi, float phi_norm,
 // (two rows of TX cells and two columns of TY + 2, corners included).
 template <int TX, int TY> constexpr int gm_ring_cells() { return 2 * TX + 2 * (TY + 2); }
 
-// Moment-form phase pass for the TX x TY column (x0, y0) over planes
-// [zb, ze). For destination plane z, every cell y of the tile and of the
+// LDS slots of phase_tile_gm_rn's link values, link-major. Link l (1..18) has a
+// row-major region of the cells that lie in the tile or send l into it:
+// x in [-max(c_x, 0), TX - 1 + max(-c_x, 0)] and likewise y, i.e.
+// (TX + |c_x|) x (TY + |c_y|) cells (the rest link stays in a register).
+// A tile cell writes every link at its own position and reads link l at its
+// position - c_l, both at a compile-time offset from one of two row bases;
+// a ring corner's links that enter neither the region nor the tile go to one
+// spare slot that nothing reads.
+template <int TX, int TY> struct GmSlots {
+    static constexpr int w(int l) { return TX + (CX[l] != 0 ? 1 : 0); }
+    static constexpr int h(int l) { return TY + (CY[l] != 0 ? 1 : 0); }
+    static constexpr int lo_x(int l) { return CX[l] > 0 ? -1 : 0; }
+    static constexpr int lo_y(int l) { return CY[l] > 0 ? -1 : 0; }
+    static constexpr int base(int l) {
+        int b = 0;
+        for (int k = 1; k < l; ++k) b += w(k) * h(k);
+        return b;
+    }
+    static constexpr int kSpare = base(Q);   // after the 18 regions
+    static constexpr int kSlots = kSpare + 1;
+    static constexpr bool inside(int l, int sx, int sy) {
+        return sx >= lo_x(l) && sx < lo_x(l) + w(l) && sy >= lo_y(l) && sy < lo_y(l) + h(l);
+    }
+    static constexpr int at(int l, int sx, int sy) { return base(l) + (sy - lo_y(l)) * w(l) + (sx - lo_x(l)); }
+    // Offsets from the row base sy * w(l) + sx of a tile cell: its own slot,
+    // and the slot of its source x - c_l.
+    static constexpr int own(int l) { return base(l) - lo_y(l) * w(l) - lo_x(l); }
+    static constexpr int src(int l) { return own(l) - CY[l] * w(l) - CX[l]; }
+};
+
+template <int TX, int TY> constexpr bool gm_slots_ok() {
+    using L = GmSlots<TX, TY>;
+    for (int l = 1; l < Q; ++l) {
+        if (L::own(l) < 0 || L::src(l) < 0) return false;
+        // Every source of the tile and every tile cell lies in the region.
+        for (int sy = -1; sy <= TY; ++sy)
+            for (int sx = -1; sx <= TX; ++sx) {
+                const bool tile = sx >= 0 && sx < TX && sy >= 0 && sy < TY;
+                const bool sends = sx + CX[l] >= 0 && sx + CX[l] < TX && sy + CY[l] >= 0 && sy + CY[l] < TY;
+                if ((tile || sends) && !L::inside(l, sx, sy)) return false;
+            }
+    }
+    return true;
+}
+
+// Moment-form phase pass with stored normals (CaseView::gm_n set: the
+// collide pass stores n beside u) for the TX x TY column (x0, y0) over
+// planes [zb, ze). For destination plane z, every cell y of the tile and of the
 // 1-cell ring around it writes to LDS the values g*_l(y) that links l carry
 // from it into plane z (from its plane z - c_z(l)), computed with
 // collisionPhase's own operations (phase_post, lbm.cu:319-326); each
@@ -431,6 +509,308 @@ __device__ __forceinline__ void phase_tile_gm(const CaseView& c, int x0, int y0,
             st32(c.phi, (unsigned)z * (unsigned)G.nxy + own, sum);
         }
         __syncthreads();   // s_g is rewritten by the next plane
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    for (int z = zb; z < ze; z += 6) {
+        iter(z, I0{}, I1{}, I2{}, I0{});
+        if (z + 1 < ze) iter(z + 1, I1{}, I2{}, I0{}, I1{});
+        if (z + 2 < ze) iter(z + 2, I2{}, I0{}, I1{}, I0{});
+        if (z + 3 < ze) iter(z + 3, I0{}, I1{}, I2{}, I1{});
+        if (z + 4 < ze) iter(z + 4, I1{}, I2{}, I0{}, I0{});
+        if (z + 5 < ze) iter(z + 5, I2{}, I0{}, I1{}, I1{});
+    }
+}
+
+// Moment-form phase pass with recomputed normals (CaseView::gm_n null) for
+// the TX x TY column (x0, y0) over planes [zb, ze). For destination plane z, every cell y of the tile and of the
+// 1-cell ring around it writes to LDS the values g*_l(y) that links l carry
+// from it into plane z (from its plane z - c_z(l)), computed with
+// collisionPhase's own operations (phase_post, lbm.cu:319-326) from its phi,
+// its u and its normal n; each destination then sums its 19 values in link
+// order, so the result carries the stored populations' bits.
+//   n is not state: it is recomputed here with gradCalc's operations from
+// the same phi the collide pass computed it from (this pass's input), so a
+// step moves phi and u (16 B per cell) instead of phi, u and n (28 B), and
+// the collide pass stores u only. phi planes with a 2-cell halo are staged
+// in a 3-slot LDS ring (a plane is loaded once per chunk plus the chunk's
+// 4 boundary planes).
+//   Each link value is computed once. In-plane opposite links l, l'
+// (c_l' = -c_l) of a cell share their work: their dot products are exact
+// negatives (IEEE rounding is sign-symmetric), so one evaluation gives both.
+//   Thread (tx, ty) owns tile cell (tx, ty): a 3-plane register window of
+// its u (z, z+1, z+2 in flight) and its n on planes z, z+1. On plane z it
+// computes n on plane z+1, writes its in-plane links and its c_z = -1 links
+// from plane z+1, and computes its c_z = +1 links from plane z, held (5
+// registers) for plane z+1. Threads tid < gm_ring_cells also own one ring
+// cell, of which they compute the normal and only the five links that enter
+// the tile; what later planes need of it is held (5 values). Slots are
+// compile-time (the z loop is unrolled by 6, the period of the window and of
+// the ring's held values). Must be called by all TX * TY threads (it contains
+// barriers).
+template <int TX, int TY>
+__device__ __forceinline__ void phase_tile_gm_rn(const CaseView& c, int x0, int y0, int zb, int ze) {
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 4, PY = TY + 4, PN = PX * PY;   // phi plane + 2-halo
+    constexpr int KP = (PN + NT - 1) / NT;
+    constexpr int NR = gm_ring_cells<TX, TY>();
+    using SL = GmSlots<TX, TY>;
+    static_assert(NR <= NT, "a lane per ring cell");
+    static_assert(gm_slots_ok<TX, TY>(), "link slots");
+    static_assert(SL::kSlots < 16384, "LDS offsets");
+    __shared__ float s_g[SL::kSlots];
+    __shared__ float s_phi[3][PN];
+
+    const Geo& G = c.geo;
+    const Derived d(c.phys);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const float w1 = c.phys.w[1], w2 = c.phys.w[2];
+    const unsigned fbytes = (unsigned)(G.n * 4);
+    const Field phi_f(c.phi_prev, fbytes), ux_f(c.gm_u[0], fbytes), uy_f(c.gm_u[1], fbytes),
+        uz_f(c.gm_u[2], fbytes);
+    auto plane4 = [&](int z) { return (unsigned)(wrapmod(z, G.nz) * G.nxy) * 4u; };
+    auto fetch_u = [&](float (&u)[3], unsigned cell4, unsigned z4) {
+        u[0] = ux_f.ld(cell4, z4);
+        u[1] = uy_f.ld(cell4, z4);
+        u[2] = uz_f.ld(cell4, z4);
+    };
+
+    // In-plane byte offsets of the staged phi cells this thread loads.
+    unsigned poff[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        const int e = tid + q * NT;
+        unsigned off = 0;
+        if (e < PN) {
+            const int lx = e % PX, ly = e / PX;
+            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx)) * 4u;
+        }
+        poff[q] = off;
+    }
+    auto load_phi = [&](int z, float* dst) {
+        const unsigned z4 = plane4(z);
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = phi_f.ld(poff[q], z4);
+        }
+    };
+    // Software-pipelined staging inside the z loop: fetch_phi() loads plane
+    // z+3 into registers during plane z, put_phi() writes it to its slot once
+    // every read of plane z (the slot's previous plane) is done.
+    float preg[KP];
+    auto fetch_phi = [&](int z) {
+        const unsigned z4 = plane4(z);
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) preg[q] = phi_f.ld(poff[q], z4);
+        }
+    };
+    auto put_phi = [&](float* dst) {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = preg[q];
+        }
+    };
+    // n of the cell staged at pc on the plane whose phi is p0 (pm below, pp
+    // above), as the collide pass computes it; (gx, gy, gz) its lattice
+    // coordinates for the ghost-layer test.
+    auto normal = [&](int pc, int gx, int gy, int gz, const float* pm, const float* p0, const float* pp,
+                      float (&n)[3]) {
+        if (c.bc == MCB_BC_REFERENCE && ref_face(G, gx, gy, gz)) {
+            n[0] = n[1] = n[2] = 0.0f;
+            return;
+        }
+        auto nb = [&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            return pick<CZ[l]>(pm, p0, pp)[pc + CY[l] * PX + CX[l]];
+        };
+        float ind;
+        grad_normal(nb, d, n[0], n[1], n[2], ind);
+    };
+
+    // ---- tile cell
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    const int pc_own = (ty + 2) * PX + tx + 2;
+    // Cells past the lattice edge (partial tiles) take their periodic image.
+    const unsigned own4 = (unsigned)(wrapmod(y, G.ny) * G.nx + wrapmod(x, G.nx)) * 4u;
+    const int row0 = ty * TX + tx, row1 = ty * (TX + 1) + tx;   // row bases of widths TX, TX + 1
+    auto own_slot = [&](auto lc) -> float& {
+        constexpr int l = decltype(lc)::value;
+        return s_g[(SL::w(l) == TX ? row0 : row1) + SL::own(l)];
+    };
+    float u[3][3];     // window: u on planes by slot
+    float nn[2][3];    // n on planes by parity
+    float hold[5];     // c_z = +1 links for plane z+1, from plane z
+    // ---- ring cell (side 0: row y0-1, 1: row y0+TY, 2: column x0-1, 3: column x0+TX)
+    const bool ring = tid < NR;
+    int hx, hy, side;
+    if (tid < TX) { hx = tid + 1; hy = 0; side = 0; }
+    else if (tid < 2 * TX) { hx = tid - TX + 1; hy = TY + 1; side = 1; }
+    else if (tid < 2 * TX + TY + 2) { hx = 0; hy = tid - 2 * TX; side = 2; }
+    else { hx = TX + 1; hy = min(tid - 2 * TX - TY - 2, TY + 1); side = 3; }
+    const unsigned roff4 = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx)) * 4u;
+    const int rgx = x0 - 1 + hx, rgy = y0 - 1 + hy;
+    const int pc_ring = (hy + 1) * PX + hx + 1;
+    // LDS slots of the side's five links at this cell.
+    int rslot[5];
+    {
+        const int sx = hx - 1, sy = hy - 1;
+        auto fill = [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            auto one = [&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int l = RING_LINK[s][k];
+                rslot[k] = SL::inside(l, sx, sy) ? SL::at(l, sx, sy) : SL::kSpare;
+            };
+            one(std::integral_constant<int, 0>{});
+            one(std::integral_constant<int, 1>{});
+            one(std::integral_constant<int, 2>{});
+            one(std::integral_constant<int, 3>{});
+            one(std::integral_constant<int, 4>{});
+        };
+        if (side == 0) fill(std::integral_constant<int, 0>{});
+        else if (side == 1) fill(std::integral_constant<int, 1>{});
+        else if (side == 2) fill(std::integral_constant<int, 2>{});
+        else fill(std::integral_constant<int, 3>{});
+    }
+    const bool ry = side < 2;                     // the side's axis is y
+    const float rs = (side & 1) ? -1.0f : 1.0f;   // and its sign
+    float rin[3];      // in-plane links of the ring cell for the next plane
+    float rhold[2];    // its c_z = +1 link for planes by parity
+    // From the ring cell's plane p: in-plane links (for plane p) into rin,
+    // c_z = -1 link (for plane p-1) returned, c_z = +1 link (plane p+1) to up.
+    auto ring_plane = [&](const PhaseCell& q, float& up) {
+        const float pu = rs * (ry ? q.uy : q.ux), qu = ry ? q.ux : q.uy;
+        const float pn = rs * (ry ? q.ny : q.nx), qn = ry ? q.nx : q.ny;
+        const float w1phi = w1 * q.phi, w2phi = w2 * q.phi;
+        rin[0] = phase_link(w1, w1phi, q.phi_norm, pu, pn);
+        rin[1] = phase_link(w2, w2phi, q.phi_norm, pu + qu, pn + qn);
+        rin[2] = phase_link(w2, w2phi, q.phi_norm, pu - qu, pn - qn);
+        up = phase_link(w2, w2phi, q.phi_norm, pu + q.uz, pn + q.nz);
+        return phase_link(w2, w2phi, q.phi_norm, pu - q.uz, pn - q.nz);
+    };
+    auto cell = [&](float phi, const float (&uu)[3], const float (&n)[3]) {
+        return phase_cell(c.phys, phi, uu[0], uu[1], uu[2], n[0], n[1], n[2]);
+    };
+
+    // Prologue. Slot of plane p: (p - zb) mod 3 for the phi ring and the u
+    // window (planes zb-2 and zb-1 pass through slots 1 and 2); parity
+    // (p - zb) mod 2 for n and the ring's held c_z = +1 link.
+    load_phi(zb - 2, s_phi[1]);
+    load_phi(zb - 1, s_phi[2]);
+    load_phi(zb, s_phi[0]);
+    fetch_u(u[2], own4, plane4(zb - 1));
+    fetch_u(u[0], own4, plane4(zb));
+    fetch_u(u[1], own4, plane4(zb + 1));
+    float rp[3];   // the ring cell's u on a prologue plane
+    if (ring) fetch_u(rp, roff4, plane4(zb - 1));
+    __syncthreads();
+    {   // plane zb-1: the tile cell's c_z = +1 links, the ring cell's
+        float nm[3];
+        normal(pc_own, x, y, zb - 1, s_phi[1], s_phi[2], s_phi[0], nm);
+        const PhaseCell pm = cell(s_phi[2][pc_own], u[2], nm);
+        int k = 0;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (CZ[l] > 0) { hold[k] = phase_post<l>(pm, c.phys); ++k; }
+        });
+        if (ring) {
+            float rn[3];
+            normal(pc_ring, rgx, rgy, zb - 1, s_phi[1], s_phi[2], s_phi[0], rn);
+            ring_plane(cell(s_phi[2][pc_ring], rp, rn), rhold[0]);
+        }
+    }
+    __syncthreads();
+    load_phi(zb + 1, s_phi[1]);
+    if (ring) fetch_u(rp, roff4, plane4(zb));
+    __syncthreads();
+    {   // plane zb: the tile cell's n, the ring cell's in-plane and c_z = +1 links
+        normal(pc_own, x, y, zb, s_phi[2], s_phi[0], s_phi[1], nn[0]);
+        if (ring) {
+            float rn[3];
+            normal(pc_ring, rgx, rgy, zb, s_phi[2], s_phi[0], s_phi[1], rn);
+            ring_plane(cell(s_phi[0][pc_ring], rp, rn), rhold[1]);   // leaves plane zb's in-plane links in rin
+        }
+    }
+    __syncthreads();
+    load_phi(zb + 2, s_phi[2]);
+    __syncthreads();
+
+    auto iter = [&](int z, auto p0, auto pp, auto pn, auto ph) {
+        constexpr int P0 = decltype(p0)::value, PP = decltype(pp)::value, PN = decltype(pn)::value;
+        constexpr int PH = decltype(ph)::value;
+        float ru[3];
+        if (ring) fetch_u(ru, roff4, plane4(z + 1));   // used after the tile cell's work
+        if (z + 1 < ze) {                              // read by the next plane
+            fetch_u(u[PN], own4, plane4(z + 2));
+            fetch_phi(z + 3);
+        }
+        // n on plane z+1 (phi planes z, z+1, z+2 in slots P0, PP, PN)
+        normal(pc_own, x, y, z + 1, s_phi[P0], s_phi[PP], s_phi[PN], nn[PH ^ 1]);
+        {   // c_z = +1 links from plane z-1, computed on the previous plane
+            int k = 0;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] > 0) { own_slot(lc) = hold[k]; ++k; }
+            });
+        }
+        float g0;
+        {   // in-plane links from plane z
+            const PhaseCell c0 = cell(s_phi[P0][pc_own], u[P0], nn[PH]);
+            g0 = phase_post<0>(c0, c.phys);
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (l != 0 && CZ[l] == 0 && l % 2 == 1) {
+                    static_assert(CX[l + 1] == -CX[l] && CY[l + 1] == -CY[l] && CZ[l + 1] == 0, "pair");
+                    const float wl = c.phys.w[WC[l]];
+                    const float ud = cdot<l>(c0.ux, c0.uy, c0.uz) * INV_CSSQ;
+                    const float wa = wl * (c0.phi_norm * cdot<l>(c0.nx, c0.ny, c0.nz));
+                    const float wphi = wl * c0.phi;
+                    own_slot(lc) = wphi * (1.0f + ud) + wa;
+                    own_slot(std::integral_constant<int, l + 1>{}) = wphi * (1.0f + -ud) + -wa;
+                }
+            });
+            int k = 0;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] > 0) { hold[k] = phase_post<l>(c0, c.phys); ++k; }
+            });
+        }
+        {   // c_z = -1 links from plane z+1
+            const PhaseCell cp = cell(s_phi[PP][pc_own], u[PP], nn[PH ^ 1]);
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (CZ[l] < 0) own_slot(lc) = phase_post<l>(cp, c.phys);
+            });
+        }
+        if (ring) {
+            float rn[3];
+            normal(pc_ring, rgx, rgy, z + 1, s_phi[P0], s_phi[PP], s_phi[PN], rn);
+            s_g[rslot[3]] = rhold[PH];
+            s_g[rslot[0]] = rin[0];
+            s_g[rslot[1]] = rin[1];
+            s_g[rslot[2]] = rin[2];
+            s_g[rslot[4]] = ring_plane(cell(s_phi[PP][pc_ring], ru, rn), rhold[PH]);
+        }
+        __syncthreads();
+        if (z + 1 < ze) put_phi(s_phi[P0]);   // plane z+3 replaces plane z
+        if (valid) {
+            float sum = 0.0f;
+            sum += g0;
+            for_links([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                if constexpr (l != 0) sum += s_g[(SL::w(l) == TX ? row0 : row1) + SL::src(l)];
+            });
+            // Ghost-layer faces take boundaryConditions' copy instead.
+            if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z)) sum = ld32(c.phi_prev, ref_clamped(G, x, y, z));
+            st32(c.phi, (unsigned)z * (unsigned)G.nxy + own4 / 4u, sum);
+        }
+        __syncthreads();   // s_g is rewritten by the next plane, phi slot P0 read
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -937,13 +1317,16 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                 });
                 if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
                     // Moment form: keep what collisionPhase needs (phic is the
-                    // phi buffer of this step, the next step's phi_prev).
+                    // phi buffer of this step, the next step's phi_prev): u,
+                    // and n unless the next phase pass recomputes it from phi.
                     Field(c.gm_u[0], fb).st(me4, z1 * 4u, m.ux);
                     Field(c.gm_u[1], fb).st(me4, z1 * 4u, m.uy);
                     Field(c.gm_u[2], fb).st(me4, z1 * 4u, m.uz);
-                    Field(c.gm_n[0], fb).st(me4, z1 * 4u, ncx);
-                    Field(c.gm_n[1], fb).st(me4, z1 * 4u, ncy);
-                    Field(c.gm_n[2], fb).st(me4, z1 * 4u, ncz);
+                    if (c.gm_n[0]) {
+                        Field(c.gm_n[0], fb).st(me4, z1 * 4u, ncx);
+                        Field(c.gm_n[1], fb).st(me4, z1 * 4u, ncy);
+                        Field(c.gm_n[2], fb).st(me4, z1 * 4u, ncz);
+                    }
                     return;
                 }
                 for_links([&](auto lc) {

@@ -51,7 +51,7 @@ def test_abi_version():
 @pytest.mark.parametrize("field,value", [
     ("nx", 2), ("ny", 0), ("nz", -4), ("tau_f", 0.5), ("tau_f", float("nan")), ("tau_g", 0.4),
     ("sigma", -1.0), ("sigma", float("inf")), ("bc_mode", 7), ("weight_set", 3), ("storage", 9),
-    ("path", 5), ("device", -1),
+    ("path", 5), ("device", -1), ("normals", 3), ("normals", -1),
 ])
 def test_validate_rejects(field, value):
     p = mcb.make_params(16, 16, 16)
@@ -113,7 +113,18 @@ def test_footprint():
     assert fused >= 4 * 19 * n * 4                      # f and g, two buffers each
     p.tau_g = 1.0
     moments = mcb.lib().mcb_footprint_bytes(ctypes.byref(p))
-    assert 2 * 19 * n * 4 + 18 * n * 4 <= moments < fused   # g as phi, u, n
+    assert 2 * 19 * n * 4 + 15 * n * 4 <= moments < fused   # g as phi (two buffers), u (and n)
+    # normals: AUTO stores n at 64^3 (short phase chunks) and recomputes it at
+    # 256^3; RECOMPUTED and STORED force either.
+    p.normals = mcb.NORMALS_RECOMPUTED
+    assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) == moments - 3 * (n + 256) * 4
+    p.normals = mcb.NORMALS_STORED
+    assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) == moments
+    n2 = 256 ** 3
+    p256 = mcb.make_params(256, 256, 256)
+    auto = mcb.lib().mcb_footprint_bytes(ctypes.byref(p256))
+    p256.normals = mcb.NORMALS_STORED
+    assert auto == mcb.lib().mcb_footprint_bytes(ctypes.byref(p256)) - 3 * (n2 + 256) * 4
     p.path = mcb.PATH_STAGED
     assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) >= 3 * 19 * 64 ** 3 * 4
 

@@ -171,6 +171,40 @@ def test_batch_case_resumes_in_another_batch(gpu_lib, storage):
         b2.step(2)
 
 
+def test_moment_form_images_cross_normal_modes(gpu_lib):
+    """An FP32 moment-form image holds phi and u, not the normal n (gradCalc
+    of that phi): a context that stores n rebuilds it on loading, one that
+    recomputes n needs nothing, and a batch case (n stored) resumes from a
+    single context's image; every continuation is bit for bit the run that
+    never stopped."""
+    mcb = gpu_lib
+    modes = (mcb.NORMALS_STORED, mcb.NORMALS_RECOMPUTED)
+    ref = None
+    for src in modes:
+        p = mcb.make_params(*SHAPE, tau_f=0.6, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=src)
+        with mcb.Lattice(p) as a:
+            a.init_bubbles(BUB, 3.0)
+            a.step(7)
+            image = a.save_state()
+            a.step(9)
+            got = snapshot(a)
+        if ref is None:
+            ref = got
+        assert_same(dict(ref), got, f"normals {src} straight through")
+        for dst in modes:
+            p.normals = dst
+            b = mcb.Lattice(p)
+            b.load_state(image)
+            b.step(9)
+            assert_same(dict(ref), snapshot(b), f"image of normals {src} into normals {dst}")
+            b.close()
+        with mcb.Batch(mcb.STORE_FP32) as bt:
+            cid = bt.add_state(image)
+            bt.step(9)
+            for name in ("f", "g"):
+                assert np.array_equal(bt.read(cid, name), ref[name]), f"batch from normals {src}: {name}"
+
+
 def test_state_images_of_other_cases_are_refused(gpu_lib):
     mcb = gpu_lib
     a = mcb.Lattice(params(mcb, mcb.STORE_FP32, 1.0))

@@ -41,13 +41,19 @@ def assert_parity(got, want, label, exact=True):
             assert nbad == 0, f"{label}: {k} differs from the oracle in {nbad} cells (rel L2 {err:.3e})"
 
 
-@pytest.mark.parametrize("path", ["fused", "staged"])
+# Fused FP32 at tau_g = 1 keeps g in moment form with the interface normal
+# either stored by the collide pass or recomputed by the phase pass.
+NORMALS = {"n-stored": 1, "n-recomputed": 2}
+
+
+@pytest.mark.parametrize("path", ["fused", "fused-n-recomputed", "staged"])
 @pytest.mark.parametrize("shape,tau_g", [((32, 32, 32), 1.0), ((48, 40, 36), 0.8), ((20, 24, 17), 1.0)])
 def test_periodic_short_bitexact(gpu_lib, path, shape, tau_g):
     mcb = gpu_lib
     nx, ny, nz = shape
     p = mcb.make_params(nx, ny, nz, tau_f=0.55, tau_g=tau_g, sigma=0.1,
-                        path=mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED)
+                        path=mcb.PATH_STAGED if path == "staged" else mcb.PATH_FUSED,
+                        normals=mcb.NORMALS_RECOMPUTED if path == "fused-n-recomputed" else mcb.NORMALS_STORED)
     bubbles = [(nx * 0.5 + 0.3, ny * 0.5 - 1.2, nz * 0.5 + 0.7, min(shape) * 0.25)]
     lat, o, got, want = run_pair(mcb, p, bubbles, 20)
     assert_parity(got, want, f"{path} {shape} tau_g={tau_g}")
@@ -76,23 +82,26 @@ def test_cfg0_1000_steps(gpu_lib, path):
     lat.close()
 
 
-def test_moment_form_1000_steps(gpu_lib):
-    """tau_g = 1 keeps g as (phi, u, n) and recomputes g* in the phase pass:
-    1000 steps stay bit-identical to the transcription."""
+@pytest.mark.parametrize("normals", list(NORMALS))
+def test_moment_form_1000_steps(gpu_lib, normals):
+    """tau_g = 1 keeps g as (phi, u, n) and recomputes g* in the phase pass,
+    with n stored or itself recomputed from phi: 1000 steps stay
+    bit-identical to the transcription."""
     mcb = gpu_lib
-    p = mcb.make_params(48, 48, 48, tau_f=0.55, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED)
+    p = mcb.make_params(48, 48, 48, tau_f=0.55, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=NORMALS[normals])
     bubbles = [(24.3, 23.6, 24.1, 10.0)]
     lat, o, got, want = run_pair(mcb, p, bubbles, 1000)
-    assert_parity(got, want, "moment form 48^3 x 1000")
+    assert_parity(got, want, f"moment form ({normals}) 48^3 x 1000")
     lat.close()
 
 
-def test_moment_form_populations(gpu_lib):
+@pytest.mark.parametrize("normals", list(NORMALS))
+def test_moment_form_populations(gpu_lib, normals):
     """g read from the moment form is the reference layout, before and after
     steps; writing g switches the context to stored populations, bit for bit."""
     mcb = gpu_lib
     shape = (24, 20, 16)
-    p = mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, sigma=0.08, path=mcb.PATH_FUSED)
+    p = mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, sigma=0.08, path=mcb.PATH_FUSED, normals=NORMALS[normals])
     bubbles = [(12.0, 10.5, 8.0, 5.0)]
     o = oracle_for(p)
     o.init(f32_bubbles(bubbles))
@@ -117,23 +126,25 @@ def test_moment_form_populations(gpu_lib):
 
 @pytest.mark.parametrize("tau_g", [0.8, 1.0])
 def test_fused_tilings_agree(gpu_lib, tau_g):
-    """Every tile shape and z-chunk gives the same bits."""
+    """Every tile shape and z-chunk gives the same bits; at tau_g = 1 with the
+    normals stored and recomputed alike."""
     mcb = gpu_lib
-    p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=tau_g, sigma=0.05, path=mcb.PATH_FUSED)
     bubbles = [(20.5, 17.0, 14.2, 7.5)]
     ref = None
-    for tx, ty in mcb.tilings():
-        for zc in (1, 5, 28):
-            lat = mcb.Lattice(p)
-            lat.init_bubbles(bubbles)
-            lat.set_tiling(tx, ty, zc)
-            lat.step(7)
-            obs = lat.observables()
-            lat.close()
-            if ref is None:
-                ref = obs
-            for k in OBS:
-                assert np.array_equal(obs[k], ref[k]), f"tiling {tx}x{ty} zc={zc}: {k} differs"
+    for normals in (list(NORMALS.values()) if tau_g == 1.0 else [0]):
+        p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=tau_g, sigma=0.05, path=mcb.PATH_FUSED, normals=normals)
+        for tx, ty in mcb.tilings():
+            for zc in (1, 5, 28):
+                lat = mcb.Lattice(p)
+                lat.init_bubbles(bubbles)
+                lat.set_tiling(tx, ty, zc)
+                lat.step(7)
+                obs = lat.observables()
+                lat.close()
+                if ref is None:
+                    ref = obs
+                for k in OBS:
+                    assert np.array_equal(obs[k], ref[k]), f"normals {normals}, tiling {tx}x{ty} zc={zc}: {k} differs"
 
 
 def test_populations_roundtrip_and_layout(gpu_lib):
@@ -171,7 +182,7 @@ def test_populations_roundtrip_and_layout(gpu_lib):
 REF_SHAPES = [(24, 20, 40), (37, 29, 33)]   # the second has partial tiles in x and y
 
 
-@pytest.mark.parametrize("path", ["fused", "staged"])
+@pytest.mark.parametrize("path", ["fused", "fused-n-recomputed", "staged"])
 @pytest.mark.parametrize("shape", REF_SHAPES)
 def test_reference_mode(gpu_lib, path, shape):
     """Ghost-layer boundaries of the reference against the oracle: the fused
@@ -182,7 +193,8 @@ def test_reference_mode(gpu_lib, path, shape):
     mcb = gpu_lib
     nx, ny, nz = shape
     p = mcb.make_params(nx, ny, nz, bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL, tau_f=0.505,
-                        tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED if path == "fused" else mcb.PATH_STAGED)
+                        tau_g=1.0, sigma=0.1, path=mcb.PATH_STAGED if path == "staged" else mcb.PATH_FUSED,
+                        normals=mcb.NORMALS_RECOMPUTED if path == "fused-n-recomputed" else mcb.NORMALS_STORED)
     bubbles = [(nx * 0.5, ny * 0.5, nz * 0.5, min(shape) * 0.3)]
     lat = mcb.Lattice(p)
     lat.init_bubbles(bubbles)
@@ -205,15 +217,17 @@ def test_reference_mode(gpu_lib, path, shape):
     lat.close()
 
 
-def test_reference_mode_fused_write_f_and_resume(gpu_lib):
+@pytest.mark.parametrize("normals", list(NORMALS))
+def test_reference_mode_fused_write_f_and_resume(gpu_lib, normals):
     """Fused ghost-layer path: F written through the ABI feeds the next
     iteration as the reference's d_f does (face populations included once),
-    and a state image resumes the run bit for bit; compared with the oracle
-    given the same F."""
+    and a state image resumes the run bit for bit, also in a context that
+    keeps the normals the other way; compared with the oracle given the
+    same F."""
     mcb = gpu_lib
     nx, ny, nz = 26, 22, 30
     p = mcb.make_params(nx, ny, nz, bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL, tau_f=0.55,
-                        tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED)
+                        tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=NORMALS[normals])
     bubbles = [(13.0, 11.0, 15.0, 6.5)]
     lat = mcb.Lattice(p)
     lat.init_bubbles(bubbles)
@@ -235,18 +249,20 @@ def test_reference_mode_fused_write_f_and_resume(gpu_lib):
     o.macros()
     for k in OBS:
         assert np.array_equal(got[k], o.field(k)), k
-    with mcb.Lattice(p) as lat2:
-        lat2.load_state(image)
-        lat2.step(5)
-        got2 = lat2.observables()
-    for k in OBS:
-        assert np.array_equal(got2[k], got[k]), f"resumed {k}"
+    for other in NORMALS.values():
+        p.normals = other
+        with mcb.Lattice(p) as lat2:
+            lat2.load_state(image)
+            lat2.step(5)
+            got2 = lat2.observables()
+        for k in OBS:
+            assert np.array_equal(got2[k], got[k]), f"resumed (normals {other}) {k}"
     with pytest.raises(mcb.McbError):
         lat.read("g")           # the reference layout of g is kept by the staged path only
     lat.close()
 
 
-@pytest.mark.parametrize("kind", ["moments", "populations", "reference"])
+@pytest.mark.parametrize("kind", ["moments", "moments-n-recomputed", "populations", "reference"])
 def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
     """mcb_step replays pairs of fused steps from a captured hipGraph. Each
     replay, with eager work (observables, a field readout) and host
@@ -258,6 +274,8 @@ def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
         kw["tau_g"] = 0.8
     if kind == "reference":
         kw.update(bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL)
+    if kind == "moments-n-recomputed":
+        kw["normals"] = mcb.NORMALS_RECOMPUTED
     p = mcb.make_params(30, 26, 22, **kw)
     bubbles = [(15.2, 13.1, 11.3, 6.0)]
     g, e = mcb.Lattice(p), mcb.Lattice(p)
