@@ -38,6 +38,9 @@ B_ALG_PHASE = 19 * 4 + 4              # k_phase: g read, phi write: 80 B per cel
 # instead of being stored.
 B_ALG_COLLIDE_GM = 2 * 19 * 4 + 3 * 4 + 4   # f read, f write, u write, phi read: 168 B per cell
 B_ALG_PHASE_GM = 4 * 4 + 4                  # phi, u read; phi write: 20 B per cell
+# f in moment form as well (the 13 values collisionFluid reads, u among them):
+# the collide pass reads a cell's 13 moments and phi, writes its 13 moments.
+B_ALG_COLLIDE_FM = 2 * 13 * 4 + 4           # 108 B per cell
 METRIC = "MLUPS at 256³ + % of HBM roofline; serving p50/p99 latency vs plain solves"
 
 
@@ -131,6 +134,28 @@ def reference_preset(mcb, steps, warmup):
         out[name] = {"mlups": round(cells / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 4)}
     out["fused_vs_staged"] = round(out["fused"]["mlups"] / out["staged"]["mlups"], 3)
     return out
+
+
+def f_moments(mcb, case, steps, warmup):
+    """The same 256^3 case with f kept as the 13 moments collisionFluid reads
+    (rho, u, Pi^neq, F) instead of 19 populations: bit-identical results,
+    108 B per cell in the collide pass instead of 168."""
+    cells = case.params.nx * case.params.ny * case.params.nz
+    lat = mcb.Lattice.from_case(case, path=mcb.PATH_FUSED, f_form=mcb.F_MOMENTS)
+    lat.step(warmup)
+    t_phase, t_collide = lat.kernel_times(steps)
+    ms = lat.step_timed(steps) / steps
+    lat.observables()
+    s, k, finite = lat.reduce()
+    tiling = lat.tiling()
+    lat.close()
+    mlups = cells / (ms * 1e-3) / 1e6
+    ph, co = t_phase / steps, t_collide / steps
+    return {"mlups": round(mlups, 1), "ms_per_step": round(ms, 4), "tiling": list(tiling),
+            "b_moved_per_lu": B_ALG_COLLIDE_FM + B_ALG_PHASE_GM,
+            "k_phase_ms": round(ph, 5), "k_collide_ms": round(co, 5),
+            "k_collide_gbs": round(B_ALG_COLLIDE_FM * cells / (co * 1e-3) / 1e9, 1),
+            "sum_phi": s, "finite": finite}
 
 
 def storage_form(mcb, case, storage, steps, warmup):
@@ -246,6 +271,8 @@ def main():
                     help="only time the 128^3 tau_g = 0.8 case (g as populations; profiling)")
     ap.add_argument("--reference-only", action="store_true",
                     help="only time the reference preset on the fused and staged kernels (profiling)")
+    ap.add_argument("--f-moments", action="store_true",
+                    help="only time the 256^3 case with f in moment form (profiling)")
     ap.add_argument("--batching-only", action="store_true",
                     help="only the 64-case batch section (profiling); --steps sets its step count")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -276,6 +303,10 @@ def main():
         return
     if args.batching_only:
         print(json.dumps({"batching": batching(mcb, steps=args.steps)}), flush=True)
+        return
+    if args.f_moments:
+        print(json.dumps({"f_moments": f_moments(mcb, mcb.case_preset("cfg2", args.seed), args.steps, args.warmup)}),
+              flush=True)
         return
     case = mcb.case_preset("cfg2", args.seed)
     p = case.params
@@ -353,6 +384,8 @@ def main():
         for name, st in (("fp16_storage", mcb.STORE_FP16), ("bf16_storage", mcb.STORE_BF16)):
             out[name] = storage_form(mcb, case, st, args.steps, args.warmup)
             out[name]["vs_plain"] = round(out[name]["mlups"] / out["value"], 3)
+        out["f_moments"] = f_moments(mcb, case, args.steps, args.warmup)
+        out["f_moments"]["vs_plain"] = round(out["f_moments"]["mlups"] / out["value"], 3)
         out["stored_g_128"] = stored_g(mcb, args.steps, args.warmup)
         out["reference_preset"] = reference_preset(mcb, args.steps, args.warmup)
         out["batching"] = batching(mcb)

@@ -72,6 +72,17 @@ typedef enum mcb_path { MCB_PATH_AUTO = 0, MCB_PATH_FUSED = 1, MCB_PATH_STAGED =
  * paths and forms ignore it. */
 typedef enum mcb_normals { MCB_NORMALS_AUTO = 0, MCB_NORMALS_STORED = 1, MCB_NORMALS_RECOMPUTED = 2 } mcb_normals;
 
+/* How the FUSED FP32 path keeps f between steps. POPULATIONS: the 19
+ * post-collision populations (pull form). MOMENTS: the 13 values
+ * collisionFluid reads (src/lbm.cu:258-287: rho, u, Pi^neq, F of each cell's
+ * last collision); the collide pass rebuilds every f*_l from them with
+ * collisionFluid's own operations, so results are bit-identical, with 52
+ * bytes read and 52 written per cell instead of 76 + 76 (+ 12 for u). No
+ * counterpart in the reference. MOMENTS needs FUSED, FP32, tau_g = 1 and
+ * PERIODIC boundaries (MCB_ERR_UNSUPPORTED otherwise); f can be read, not
+ * written, in this form. */
+typedef enum mcb_f_form { MCB_F_POPULATIONS = 0, MCB_F_MOMENTS = 1 } mcb_f_form;
+
 typedef struct mcb_params {
     int nx, ny, nz;     /* lattice sides (>= 3; reference 64 x 64 x 256, src/var.cu:3-4) */
     int bc_mode;        /* mcb_bc_mode */
@@ -85,7 +96,8 @@ typedef struct mcb_params {
     float sharp_c;      /* interface sharpening H_SHARP_C; < 0 selects 0.15f*3.0f (src/var.cu:24) */
     int moving_wall;    /* 1: also run the reference's movingWallBC (dead code, src/lbm.cu:430-461) */
     int normals;        /* mcb_normals */
-    int reserved[6];
+    int f_form;         /* mcb_f_form */
+    int reserved[5];
 } mcb_params;
 
 typedef struct mcb_bubble {

@@ -68,6 +68,11 @@ struct mcb_ctx {
     float* phis[2] = {nullptr, nullptr};
     float* gmu[3] = {nullptr, nullptr, nullptr};
     float* gmn[3] = {nullptr, nullptr, nullptr};   // null: the phase pass recomputes n
+    // f in moment form (MCB_F_MOMENTS): f[0], f[1] hold 13 planes each, u is
+    // their planes 1..3, and F/G readouts gather into fscratch (19 planes,
+    // allocated on first use).
+    bool fm = false;
+    float* fscratch = nullptr;
     long long steps = 0;   // steps since the initial state (non-batch paths)
     // Fused path with REFERENCE boundaries: a written F put arbitrary values
     // into the face populations of f[ref_reset_f], which the next step reads
@@ -150,7 +155,7 @@ bool fused_reference_ok(const mcb_params& p) {
 int resolve_path(const mcb_params& p) {
     if (p.path == MCB_PATH_STAGED) return MCB_PATH_STAGED;
     if (p.path == MCB_PATH_FUSED) return MCB_PATH_FUSED;
-    if (p.storage != MCB_STORE_FP32) return MCB_PATH_FUSED;
+    if (p.storage != MCB_STORE_FP32 || p.f_form == MCB_F_MOMENTS) return MCB_PATH_FUSED;
     return (p.bc_mode == MCB_BC_PERIODIC || fused_reference_ok(p)) && fused_fits(p) ? MCB_PATH_FUSED
                                                                                     : MCB_PATH_STAGED;
 }
@@ -169,7 +174,9 @@ size_t footprint(const mcb_params& p, int path) {
     const long long n = (long long)p.nx * p.ny * p.nz;
     const long long P = plane_stride(n);
     size_t floats = 0;
-    if (path == MCB_PATH_FUSED && p.tau_g == 1.0f)   // g in moment form: phi twice, u, and n unless recomputed
+    if (path == MCB_PATH_FUSED && p.f_form == MCB_F_MOMENTS)   // 13 moments A/B (u among them), phi twice, n
+        floats = 2 * 13 * (size_t)P + (ctx_recomputes_n(p) ? 12 : 15) * (size_t)P;
+    else if (path == MCB_PATH_FUSED && p.tau_g == 1.0f)   // g in moment form: phi twice, u, and n unless recomputed
         floats = 2 * 19 * (size_t)P + (ctx_recomputes_n(p) ? 15 : 18) * (size_t)P;
     else if (path == MCB_PATH_FUSED) floats = 4 * 19 * (size_t)P + 11 * (size_t)P;
     else floats = 3 * 19 * (size_t)P + 19 * (size_t)P;
@@ -202,10 +209,11 @@ CaseView fused_view(mcb_ctx* c) {
         v.phi = c->phis[c->cur];
         v.phi_prev = c->phis[c->cur ^ 1];
         for (int k = 0; k < 3; ++k) {
-            v.gm_u[k] = c->gmu[k];
+            v.gm_u[k] = c->fm ? c->f[c->cur] + (size_t)(1 + k) * c->geo.plane : c->gmu[k];
             v.gm_n[k] = c->gmn[k];
         }
     }
+    v.fm = c->fm ? 1 : 0;
     v.rho = c->rho; v.ux = c->ux; v.uy = c->uy; v.uz = c->uz;
     v.fx = c->ffx; v.fy = c->ffy; v.fz = c->ffz;
     v.nx = c->normx; v.ny = c->normy; v.nz = c->normz;
@@ -235,6 +243,7 @@ hipError_t fused_init_state(mcb_ctx* c) {
         hipError_t e = hipMemsetAsync(q, 0, pbytes, c->stream);
         if (e != hipSuccess) return e;
     }
+    if (c->fm) return launch_fm_init(c->f[0], c->geo, c->stream);
     return launch_init_pull(c->phi, c->f[0], c->g[0], c->geo, c->phys, c->stream);
 }
 
@@ -430,6 +439,8 @@ std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
         for (const auto& f : fields) parts.push_back({f.first, f.second, 1, N, N, 4, false});
         return parts;
     }
+    if (c->fm)   // the 13 moments (u among them) and the state's phi
+        return {{"fm", c->f[c->cur], kFm, N, P, 4, false}, {"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false}};
     std::vector<StatePart> parts{{"f", c->f[c->cur], 19, N, P, 4, false}};
     if (c->gm == GM_POPULATIONS) {
         parts.push_back({"g", c->g[c->cur], 19, N, P, 4, false});
@@ -500,6 +511,9 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
     if (path == MCB_PATH_FUSED && ((p->bc_mode != MCB_BC_PERIODIC && !fused_reference_ok(*p)) || !fused_fits(*p)))
         return MCB_ERR_UNSUPPORTED;
     if (p->storage != MCB_STORE_FP32 && path != MCB_PATH_FUSED) return MCB_ERR_UNSUPPORTED;
+    if (p->f_form == MCB_F_MOMENTS && (path != MCB_PATH_FUSED || p->storage != MCB_STORE_FP32 || p->tau_g != 1.0f ||
+                                       p->bc_mode != MCB_BC_PERIODIC))
+        return MCB_ERR_UNSUPPORTED;
 
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return MCB_ERR_NO_DEVICE;
@@ -539,8 +553,9 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
         // At tau_g = 1 g is kept in moment form: phi (two buffers), u and n
         // (or n recomputed by the phase pass) instead of two sets of 19 planes.
         const bool moments = p->tau_g == 1.0f;
+        c->fm = p->f_form == MCB_F_MOMENTS;
         for (int b = 0; b < 2; ++b) {
-            if ((st = alloc(c, &c->f[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
+            if ((st = alloc(c, &c->f[b], (c->fm ? kFm : 19) * (size_t)P)) != MCB_OK) return bail(st);
             if (!moments && (st = alloc(c, &c->g[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
         }
         float** scal[] = {&c->phi, &c->rho, &c->ux, &c->uy, &c->uz, &c->ffx, &c->ffy, &c->ffz,
@@ -551,8 +566,8 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
             c->gm = GM_INITIAL;
             c->phis[0] = c->phi;
             float** gmf[] = {&c->phis[1], &c->gmu[0], &c->gmu[1], &c->gmu[2]};
-            for (auto* q : gmf)
-                if ((st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
+            for (auto* q : gmf)   // with f in moment form u is among its planes
+                if ((q == &c->phis[1] || !c->fm) && (st = alloc(c, q, (size_t)P)) != MCB_OK) return bail(st);
             // n is stored beside u unless the phase pass recomputes it.
             if (!ctx_recomputes_n(*p))
                 for (int k = 0; k < 3; ++k)
@@ -572,6 +587,17 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
             if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MCB_ERR_HIP);
         }
         c->tiling = choose_tiling_single(*p);
+        if (c->fm) {
+            if (!fm_tiling_supported(c->tiling))
+                c->tiling = Tiling{p->nx >= 64 ? 64 : (p->nx >= 32 ? 32 : 16), p->nx >= 32 ? 8 : 16, c->tiling.zc};
+            // Its collide pass starts each chunk with two source planes:
+            // 32-plane chunks where the lattice keeps two workgroups per CU
+            // (256^3: 0.595 against 0.609 ms with 16).
+            const Tiling& t = c->tiling;
+            const long long layer = (long long)((p->nx + t.tx - 1) / t.tx) * ((p->ny + t.ty - 1) / t.ty);
+            if (2 * t.zc <= p->nz && t.zc < 32 && layer * ((p->nz + 2 * t.zc - 1) / (2 * t.zc)) >= 512)
+                c->tiling.zc = 2 * t.zc;
+        }
     } else {
         // The reference layout: planes of exactly n cells (src/header.cuh:21).
         c->geo.plane = n;
@@ -657,7 +683,7 @@ mcb_status ctx_reset_case(mcb_ctx* c, const mcb_params& p, const BubbleSet& s) {
     if (!c) return MCB_ERR_INVALID_ARG;
     if (host_validate(&p, &c->err) != MCB_OK) return MCB_ERR_INVALID_ARG;
     if (p.nx != c->p.nx || p.ny != c->p.ny || p.nz != c->p.nz || p.storage != c->p.storage ||
-        p.bc_mode != c->p.bc_mode || (p.tau_g == 1.0f) != (c->p.tau_g == 1.0f))
+        p.bc_mode != c->p.bc_mode || (p.tau_g == 1.0f) != (c->p.tau_g == 1.0f) || p.f_form != c->p.f_form)
         return fail(c, MCB_ERR_INVALID_ARG, "a context is re-targeted only at a case of the same shape and form");
     c->p.tau_f = p.tau_f;
     c->p.tau_g = p.tau_g;
@@ -761,15 +787,23 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
             if (field == MCB_FIELD_G && c->p.bc_mode == MCB_BC_REFERENCE)
                 return fail(c, MCB_ERR_UNSUPPORTED,
                             "g in the reference layout with ghost-layer boundaries: use the staged path");
-            // Gather into the idle A/B buffer, then copy plane by plane.
-            if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
-                MCB_HIP(c, launch_gm_to_ref(fused_view(c), c->f[c->cur ^ 1], c->stream));
+            // Gather into the idle A/B buffer (f in moment form: a scratch
+            // buffer of 19 planes), then copy plane by plane.
+            if (c->fm && !c->fscratch) {
+                mcb_status as = alloc(c, &c->fscratch, 19 * (size_t)c->geo.plane);
+                if (as != MCB_OK) return as;
+            }
+            float* idle_f = c->fm ? c->fscratch : c->f[c->cur ^ 1];
+            if (field == MCB_FIELD_F && c->fm) {
+                MCB_HIP(c, launch_fm_to_ref(fused_view(c), idle_f, c->stream));
+            } else if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
+                MCB_HIP(c, launch_gm_to_ref(fused_view(c), idle_f, c->stream));
             } else {
                 const float* src = field == MCB_FIELD_F ? c->f[c->cur] : c->g[c->cur];
-                MCB_HIP(c, launch_link_shift(src, field == MCB_FIELD_F ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1], c->geo,
-                                             +1, c->stream));
+                MCB_HIP(c, launch_link_shift(src, field == MCB_FIELD_F ? idle_f : c->g[c->cur ^ 1], c->geo, +1,
+                                             c->stream));
             }
-            const float* tmp = field == MCB_FIELD_F || c->gm != GM_POPULATIONS ? c->f[c->cur ^ 1] : c->g[c->cur ^ 1];
+            const float* tmp = field == MCB_FIELD_F || c->gm != GM_POPULATIONS ? idle_f : c->g[c->cur ^ 1];
             for (int l = 0; l < 19; ++l)
                 MCB_HIP(c, hipMemcpyAsync(host + l * cells, tmp + (size_t)l * c->geo.plane, cells * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream));
@@ -802,6 +836,8 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             float* dst = field == MCB_FIELD_F ? c->sf : c->sg;
             MCB_HIP(c, hipMemcpyAsync(dst, host, 19 * cells * sizeof(float), hipMemcpyHostToDevice, c->stream));
         } else {
+            if (c->fm)
+                return fail(c, MCB_ERR_UNSUPPORTED, "populations cannot be written with f in moment form");
             if (field == MCB_FIELD_G && c->p.bc_mode == MCB_BC_REFERENCE)
                 return fail(c, MCB_ERR_UNSUPPORTED, "writing g with ghost-layer boundaries: use the staged path");
             if (field == MCB_FIELD_G && c->gm != GM_POPULATIONS) {
@@ -916,7 +952,8 @@ mcb_status mcb_step_kernel_times(mcb_ctx* c, int nsteps, double* out_ms, int n_o
 mcb_status mcb_set_tiling(mcb_ctx* c, int tx, int ty, int zc) {
     if (!c) return MCB_ERR_INVALID_ARG;
     Tiling t{tx, ty, zc};
-    const bool ok = c->p.storage == MCB_STORE_FP32 ? tiling_supported(t) : tiling_supported16(t);
+    const bool ok = c->fm ? fm_tiling_supported(t)
+                          : (c->p.storage == MCB_STORE_FP32 ? tiling_supported(t) : tiling_supported16(t));
     if (!ok || zc > c->p.nz) return fail(c, MCB_ERR_INVALID_ARG, "unsupported tiling");
     if (c->batch && c->case_id) {
         mcb_status st = batch_status(c, c->batch->set_tiling(c->case_id, t));

@@ -264,6 +264,7 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     std::string why;
     if (host_validate(&p, &why) != MCB_OK) return fail(MCB_ERR_INVALID_ARG, why);
     if (p.bc_mode != MCB_BC_PERIODIC) return fail(MCB_ERR_UNSUPPORTED, "batched cases are periodic");
+    if (p.f_form != MCB_F_POPULATIONS) return fail(MCB_ERR_UNSUPPORTED, "batched cases keep f as populations");
     if (storage_ >= 0 && p.storage != storage_) return fail(MCB_ERR_UNSUPPORTED, "one storage form per batch");
     const long long P = plane_of(p);
     if (19LL * P * pop_bytes(p.storage) > 0xFFFFFFFFLL) return fail(MCB_ERR_UNSUPPORTED, "lattice too large");

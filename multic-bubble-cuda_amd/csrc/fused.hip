@@ -56,6 +56,17 @@ __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, un
 // 512-thread tiles is held to 80 VGPRs: three workgroups per CU (LDS ~49 KB
 // each).
 template <int TX, int TY, bool RN> constexpr int phase_gm_waves_per_eu() { return RN && TX * TY == 512 ? 6 : 1; }
+
+// Tile of the phase pass when it recomputes n (nx, ny at least its sides):
+// its ring cells each compute a normal, so the taller 32x16 tile (100 ring
+// cells per 512, in two waves) does less than the collide pass's 64x8 (148
+// per 512, three waves).
+#ifndef MCB_PHASE_RN_TX
+#define MCB_PHASE_RN_TX 32
+#endif
+#ifndef MCB_PHASE_RN_TY
+#define MCB_PHASE_RN_TY 16
+#endif
 template <int TX, int TY, bool RN>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(phase_gm_waves_per_eu<TX, TY, RN>())))
 void k_phase_gm(CaseView c) {
@@ -87,6 +98,20 @@ __global__ __launch_bounds__(PBX * PBY) void k_normals(CaseView c) {
 }
 
 // --------------------------------------------------------------- k_collide
+
+// f in moment form (collide_tile_fm): the collide pass's grid, held to 128
+// VGPRs (two 8-wave workgroups per CU).
+template <int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void k_collide_fm(CaseView c) {
+    const Geo& G = c.geo;
+    const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
+    unsigned t = xcd_order(blockIdx.x, gridDim.x);
+    const int x0 = (int)(t % gx) * TX;
+    t /= gx;
+    const int y0 = (int)(t % gy) * TY;
+    const int zb = (int)(t / gy) * c.zc;
+    collide_tile_fm<TX, TY, MODE>(c, x0, y0, zb, min(zb + c.zc, G.nz));
+}
 
 template <int TX, int TY, int MODE>
 __global__ __launch_bounds__(TX * TY) void k_collide(CaseView c) {
@@ -142,6 +167,34 @@ __global__ __launch_bounds__(PBX * PBY) void k_gm_to_ref(CaseView c, float* __re
         if (c.gm == GM_INITIAL) v = c.phys.w[WC[l]] * ld32(c.phi_prev, me);
         else v = g_from_moments<l>(c, x - CX[l], y - CY[l], z - CZ[l]);
         dst[(size_t)l * G.plane + me] = v;
+    });
+}
+
+// f in moment form: the initial state, rho = 1 and the other 12 moments 0.
+__global__ __launch_bounds__(256) void k_fm_init(float* __restrict__ fm, long long plane) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < kFm * plane; i += (long long)gridDim.x * 256)
+        fm[i] = i < plane ? 1.0f : 0.0f;
+}
+
+// f in moment form -> the reference's post-stream f layout:
+// dst_l(x) = f*_l(x - c_l), collisionFluid of the source cell's moments.
+__global__ __launch_bounds__(PBX * PBY) void k_fm_to_ref(CaseView c, float* __restrict__ dst) {
+    const Geo& G = c.geo;
+    const int x = blockIdx.x * PBX + threadIdx.x;
+    const int y = blockIdx.y * PBY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= G.nx || y >= G.ny) return;
+    const Derived d(c.phys);
+    const float* m = static_cast<const float*>(c.fsrc);
+    const unsigned me = (unsigned)(z * G.nxy + y * G.nx + x);
+    for_links([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        const unsigned src = (unsigned)(wrapmod(z - CZ[l], G.nz) * G.nxy + wrapmod(y - CY[l], G.ny) * G.nx +
+                                        wrapmod(x - CX[l], G.nx));
+        float v[kFm];
+#pragma unroll
+        for (int k = 0; k < kFm; ++k) v[k] = m[(size_t)k * G.plane + src];
+        dst[(size_t)l * G.plane + me] = fluid_post<l>(fluid_cell_of(v), c.phys, d);
     });
 }
 
@@ -261,11 +314,28 @@ dim3 cell_grid(const Geo& G) { return dim3((G.nx + PBX - 1) / PBX, (G.ny + PBY -
 const Tiling kTilings[] = {MCB_TILES_FP32(MCB_TILE_ENTRY)};
 #undef MCB_TILE_ENTRY
 
+// Tile shapes of the f-moment collide pass (the FP32 tiles choose_tiling
+// picks: 64x8, 32x8, 16x16; and 16x8).
+#define MCB_TILES_FM(X) X(64, 8) X(32, 8) X(32, 16) X(16, 16) X(16, 8)
+
 template <int TX, int TY>
 hipError_t launch_collide_t(const CaseView& c, int mode, hipStream_t s) {
     const unsigned tiles = (unsigned)((c.geo.nx + TX - 1) / TX) * (unsigned)((c.geo.ny + TY - 1) / TY) *
                            (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
     dim3 grid(tiles, 1, 1), block(TX, TY, 1);
+    if (c.fm) {
+#define MCB_FM_CASE(FX, FY)                                                                                   \
+    if constexpr (TX == FX && TY == FY) {                                                                     \
+        if (mode == COLLIDE_STEP) hipLaunchKernelGGL((k_collide_fm<TX, TY, COLLIDE_STEP>), grid, block, 0, s, c); \
+        else if (mode == COLLIDE_MACROS)                                                                      \
+            hipLaunchKernelGGL((k_collide_fm<TX, TY, COLLIDE_MACROS>), grid, block, 0, s, c);                 \
+        else return hipErrorInvalidValue;                                                                     \
+        return hipGetLastError();                                                                             \
+    }
+        MCB_TILES_FM(MCB_FM_CASE)
+#undef MCB_FM_CASE
+        return hipErrorInvalidValue;
+    }
     switch (mode) {
         case COLLIDE_STEP: hipLaunchKernelGGL((k_collide<TX, TY, COLLIDE_STEP>), grid, block, 0, s, c); break;
         case COLLIDE_STEP_BGK: hipLaunchKernelGGL((k_collide<TX, TY, COLLIDE_STEP_BGK>), grid, block, 0, s, c); break;
@@ -310,6 +380,13 @@ hipError_t launch_batch(const CaseView* cases, const WorkItem* work, unsigned n,
 
 }  // namespace
 
+bool fm_tiling_supported(const Tiling& t) {
+#define MCB_FM_ENTRY(FX, FY) if (t.tx == FX && t.ty == FY) return t.zc >= 1;
+    MCB_TILES_FM(MCB_FM_ENTRY)
+#undef MCB_FM_ENTRY
+    return false;
+}
+
 bool tiling_supported(const Tiling& t) {
     for (const auto& k : kTilings)
         if (k.tx == t.tx && k.ty == t.ty) return t.zc >= 1;
@@ -319,24 +396,37 @@ bool tiling_supported(const Tiling& t) {
 int tiling_count() { return (int)(sizeof(kTilings) / sizeof(kTilings[0])); }
 Tiling tiling_at(int i) { return kTilings[i]; }
 
-int phase_chunk(const Geo& geo, const Tiling& t) {
-    const unsigned layer = (unsigned)((geo.nx + t.tx - 1) / t.tx) * (unsigned)((geo.ny + t.ty - 1) / t.ty);
-    int zc = std::min(4 * t.zc, geo.nz);
-    while (zc > t.zc && layer * (unsigned)((geo.nz + zc - 1) / zc) < 256u) zc = (zc + 1) / 2;
-    return zc;
+Tiling phase_tiling(const Geo& geo, const Tiling& t, bool rn) {
+    Tiling p = t;
+    if (rn && geo.nx >= MCB_PHASE_RN_TX && geo.ny >= MCB_PHASE_RN_TY) {
+        p.tx = MCB_PHASE_RN_TX;
+        p.ty = MCB_PHASE_RN_TY;
+    }
+    // Chunks of 64 planes, halved (down to 8) until the pass has two
+    // workgroups per CU.
+    const unsigned layer = (unsigned)((geo.nx + p.tx - 1) / p.tx) * (unsigned)((geo.ny + p.ty - 1) / p.ty);
+    p.zc = std::min(64, geo.nz);
+    while (p.zc > 8 && layer * (unsigned)((geo.nz + p.zc - 1) / p.zc) < 512u) p.zc = (p.zc + 1) / 2;
+    return p;
 }
 
-bool phase_recomputes_n(const Geo& geo, const Tiling& t) { return phase_chunk(geo, t) >= 32; }
+bool phase_recomputes_n(const Geo& geo, const Tiling& t) { return phase_tiling(geo, t, true).zc >= 32; }
 
 hipError_t launch_phase(const CaseView& c0, const Tiling& t, hipStream_t s) {
     if (c0.gm == GM_MOMENTS) {
         CaseView c = c0;
-        c.zc = phase_chunk(c.geo, t);
-        const unsigned tiles = (unsigned)((c.geo.nx + t.tx - 1) / t.tx) * (unsigned)((c.geo.ny + t.ty - 1) / t.ty) *
-                               (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
         const bool rn = c.gm_n[0] == nullptr;
+        const Tiling p = phase_tiling(c.geo, t, rn);
+        c.zc = p.zc;
+        const unsigned tiles = (unsigned)((c.geo.nx + p.tx - 1) / p.tx) * (unsigned)((c.geo.ny + p.ty - 1) / p.ty) *
+                               (unsigned)((c.geo.nz + c.zc - 1) / c.zc);
+        if (rn && p.tx == MCB_PHASE_RN_TX && p.ty == MCB_PHASE_RN_TY) {
+            hipLaunchKernelGGL((k_phase_gm<MCB_PHASE_RN_TX, MCB_PHASE_RN_TY, true>), dim3(tiles),
+                               dim3(MCB_PHASE_RN_TX, MCB_PHASE_RN_TY, 1), 0, s, c);
+            return hipGetLastError();
+        }
 #define MCB_PHASE_GM(TX, TY)                                                                                     \
-    if (t.tx == TX && t.ty == TY) {                                                                              \
+    if (p.tx == TX && p.ty == TY) {                                                                              \
         if (rn) hipLaunchKernelGGL((k_phase_gm<TX, TY, true>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);          \
         else hipLaunchKernelGGL((k_phase_gm<TX, TY, false>), dim3(tiles), dim3(TX, TY, 1), 0, s, c);            \
         return hipGetLastError();                                                                                \
@@ -375,6 +465,18 @@ hipError_t launch_collide_batch(const CaseView* cases, const WorkItem* work, uns
 
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s) {
     hipLaunchKernelGGL(k_init_pull, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, phi, f, g, geo, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_fm_init(float* fm, const Geo& geo, hipStream_t s) {
+    const long long blocks = std::min<long long>((kFm * geo.plane + 255) / 256, 256LL * 16);
+    hipLaunchKernelGGL(k_fm_init, dim3((unsigned)blocks), dim3(256), 0, s, fm, geo.plane);
+    return hipGetLastError();
+}
+
+hipError_t launch_fm_to_ref(const CaseView& c, float* dst, hipStream_t s) {
+    if (!c.fm) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fm_to_ref, cell_grid(c.geo), dim3(PBX, PBY, 1), 0, s, c, dst);
     return hipGetLastError();
 }
 

@@ -28,6 +28,10 @@ enum CollideMode { COLLIDE_STEP = 0, COLLIDE_STEP_BGK = 1, COLLIDE_MACROS = 2 };
 //                    the first step of the moment form; phi_prev holds phi0.
 enum GMode { GM_POPULATIONS = 0, GM_MOMENTS = 1, GM_INITIAL = 2 };
 
+// Planes of f in moment form (CaseView::fm): rho, ux, uy, uz, pxx, pyy,
+// pzz, pxy, pxz, pyz, fx, fy, fz.
+constexpr int kFm = 13;
+
 struct Tiling {
     int tx, ty, zc;   // x/y tile of a workgroup, z cells marched per workgroup
 };
@@ -93,6 +97,10 @@ struct CaseView {
     float* gm_u[3];
     float* gm_n[3];
     int gm;                 // GM_POPULATIONS, GM_MOMENTS or GM_INITIAL
+    // f in moment form (FP32, g in moment form, periodic; collide_tile_fm):
+    // fsrc/fdst hold 13 planes (rho, u, Pi^neq, F of a cell's last collision)
+    // instead of 19 populations, and gm_u points at the state's u planes.
+    int fm;
     // MCB_BC_PERIODIC, or MCB_BC_REFERENCE (FP32 moment form of a single
     // case): the reference's ghost-layer boundaries (src/lbm.cu:365-425) in
     // pull form. Face cells never collide; their f*_l stays w_l, their u
@@ -128,7 +136,8 @@ struct WorkItem {
 // contiguous run per tile row and no padded lanes, which measured 1.1-1.9x
 // the 64- and 32-wide tiles for the 16-bit forms on such sides.
 #define MCB_TILES_FP32(X) \
-    X(64, 8) X(64, 4) X(32, 8) X(32, 4) X(16, 16) X(16, 8) X(48, 4) X(80, 4) X(96, 4) X(112, 4) X(144, 4) X(176, 4)
+    X(64, 8) X(64, 4) X(32, 8) X(32, 4) X(32, 16) X(16, 16) X(16, 8) X(48, 4) X(80, 4) X(96, 4) X(112, 4) X(144, 4) \
+    X(176, 4)
 #define MCB_TILES_16(X) \
     X(64, 8) X(64, 4) X(32, 8) X(16, 16) X(48, 4) X(80, 4) X(96, 4) X(112, 4) X(144, 4) X(176, 4)
 
@@ -139,10 +148,13 @@ Tiling tiling_at(int i);
 // ---- single case, FP32 populations (bit-exact form)
 // Phase pass; in moment form (gm == GM_MOMENTS) it runs on the tiles t.
 hipError_t launch_phase(const CaseView& c, const Tiling& t, hipStream_t s);
-// z-chunk of the moment-form phase pass on tiles t: four times the collide
-// pass's (each chunk also reads the planes around it), halved while the pass
-// would have fewer than 256 workgroups.
-int phase_chunk(const Geo& geo, const Tiling& t);
+// Tile and z-chunk of the moment-form phase pass for collide tiles t, rn =
+// n recomputed: the tile is t's, or the recomputing pass's own where the
+// lattice spans it; the chunk is as long as the lattice allows with two
+// workgroups per CU (64 planes, halved down to 8: each chunk also reads the
+// planes around it, and the recomputing pass starts each chunk with two
+// planes of normals).
+Tiling phase_tiling(const Geo& geo, const Tiling& t, bool rn);
 // Whether a single-case FP32 moment-form context recomputes n in the phase
 // pass (CaseView::gm_n null) rather than storing it: on chunks of 32 planes
 // or more, where the 24 bytes per cell saved outweigh the per-chunk start.
@@ -156,6 +168,13 @@ hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo
 // Moment form of g -> reference layout (post-stream g_l(x), 19 planes):
 // FP32 (phi, u) form (n recomputed), and the (phi, B) form of the 16-bit storage forms.
 hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s);
+// f in moment form: the initial state (rho = 1, every other moment 0, whose
+// rebuilt f*_l is initTensor's w_l) into the 13 planes of fm, and the
+// reference's post-stream f layout (19 planes) rebuilt from the state c.fsrc.
+hipError_t launch_fm_init(float* fm, const Geo& geo, hipStream_t s);
+hipError_t launch_fm_to_ref(const CaseView& c, float* dst, hipStream_t s);
+// Tile shapes instantiated for the f-moment collide pass.
+bool fm_tiling_supported(const Tiling& t);
 hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->
 // reference layout; sign = -1: reference layout -> pull form).

@@ -33,6 +33,7 @@ mcb_status host_validate(const mcb_params* p, std::string* why) {
     if (p->storage < MCB_STORE_FP32 || p->storage > MCB_STORE_BF16) return bad("unknown storage");
     if (p->path < MCB_PATH_AUTO || p->path > MCB_PATH_STAGED) return bad("unknown path");
     if (p->normals < MCB_NORMALS_AUTO || p->normals > MCB_NORMALS_RECOMPUTED) return bad("unknown normals");
+    if (p->f_form != MCB_F_POPULATIONS && p->f_form != MCB_F_MOMENTS) return bad("unknown f_form");
     if (!(p->tau_f > 0.5f) || !std::isfinite(p->tau_f)) return bad("tau_f must be a finite value > 0.5");
     if (!(p->tau_g > 0.5f) || !std::isfinite(p->tau_g)) return bad("tau_g must be a finite value > 0.5");
     if (!std::isfinite(p->sigma) || p->sigma < 0.0f) return bad("sigma must be finite and >= 0");

@@ -116,7 +116,7 @@ mcb_status state_image_load(hipStream_t s, const void* buf, size_t bytes, const 
 bool same_state_params(const mcb_params& a, const mcb_params& b) {
     return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.bc_mode == b.bc_mode && a.weight_set == b.weight_set &&
            a.storage == b.storage && a.tau_f == b.tau_f && a.tau_g == b.tau_g && a.sigma == b.sigma &&
-           a.sharp_c == b.sharp_c && a.moving_wall == b.moving_wall;
+           a.sharp_c == b.sharp_c && a.moving_wall == b.moving_wall && a.f_form == b.f_form;
 }
 
 }  // namespace mcb

@@ -62,7 +62,7 @@ mcb_status state_image_load(hipStream_t s, const void* buf, size_t bytes, const 
                             std::string* err);
 
 // The parameters that determine a state's meaning: everything but the
-// device and the requested path.
+// device, the requested path and the normal mode (n is not in an image).
 bool same_state_params(const mcb_params& a, const mcb_params& b);
 
 }  // namespace mcb

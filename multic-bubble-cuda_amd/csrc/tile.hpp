@@ -1348,6 +1348,360 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     }
 }
 
+// ------------------------------------------------------- f in moment form
+
+// f in moment form (the moment representation of f; no counterpart in the
+// reference): the state keeps, per cell, the 13 values collisionFluid reads
+// (lbm.cu:258-287) from its last collision instead of the 19 post-collision
+// populations, one plane each: rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz
+// (Pi^neq), fx, fy, fz. f*_l(y) is collisionFluid of those values, so the
+// collide pass rebuilds f*_l(x - c_l) from the source cell's moments with
+// collisionFluid's own operations: the same bits as stored populations.
+
+// The 13 moment planes of one buffer behind one buffer resource.
+struct FmPlanes {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned plane_bytes;
+    __device__ __forceinline__ FmPlanes(const void* base, unsigned plane_elems)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(kFm * plane_elems * 4u),
+                                              kBufferConfigWord)),
+          plane_bytes(plane_elems * 4u) {}
+    __device__ __forceinline__ float ld(int k, unsigned vb, unsigned sb) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vb, (unsigned)k * plane_bytes + sb, 0));
+    }
+    __device__ __forceinline__ void st(int k, unsigned vb, unsigned sb, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vb, (unsigned)k * plane_bytes + sb,
+                                              MCB_STORE_AUX);
+    }
+};
+
+// collisionFluid's per-cell values (fluid_cell) from the 13 stored moments.
+__device__ __forceinline__ FluidCell fluid_cell_of(const float (&v)[kFm]) {
+    Moments m;
+    m.rho = v[0];
+    m.ux = v[1];
+    m.uy = v[2];
+    m.uz = v[3];
+    m.pxx = v[4];
+    m.pyy = v[5];
+    m.pzz = v[6];
+    m.pxy = v[7];
+    m.pxz = v[8];
+    m.pyz = v[9];
+    return fluid_cell(m, v[10], v[11], v[12]);
+}
+
+// Collide pass with f in moment form (FP32, g in moment form, periodic) for
+// the TX x TY column (x0, y0) over planes [zb, ze). For destination plane z,
+// every source cell of the tile and of the 1-cell ring around it writes to
+// LDS (the link-major GmSlots layout of phase_tile_gm_rn) the values f*_l it
+// sends into plane z, each computed once from its moments: in iteration z a
+// tile cell builds its plane z+1 (collisionFluid's per-cell values, then
+// fluid_post per link), writes its c_z = -1 links at once and holds its
+// in-plane links (9, rest link first) for plane z+1 and its c_z = +1 links
+// (5) for plane z+2; a ring cell does the same for the five links it sends
+// into the tile. Cells past the lattice edge (partial tiles) act as their
+// periodic image. Each destination then reads its 19 values and continues as
+// collide_tile: normals and curvature from the LDS phi ring, momentiCalc, and
+// the 13 moments of this collision (or, with COLLIDE_MACROS, the
+// observables) stored aligned; with stored normals, n as well. Must be called
+// by all TX * TY threads of the workgroup (it contains barriers).
+template <int TX, int TY, int MODE>
+__device__ __forceinline__ void collide_tile_fm(const CaseView& c, int x0, int y0, int zb, int ze) {
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 4, PY = TY + 4, PN = PX * PY;  // phi plane + 2-halo
+    constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;  // normal plane + 1-halo
+    constexpr int KP = (PN + NT - 1) / NT;
+    constexpr int KH = (HN + NT - 1) / NT;
+    constexpr int NR = gm_ring_cells<TX, TY>();
+    using SL = GmSlots<TX, TY>;
+    static_assert(NR <= NT, "a lane per ring cell");
+    static_assert(gm_slots_ok<TX, TY>(), "link slots");
+    static_assert(MODE == COLLIDE_STEP || MODE == COLLIDE_MACROS, "f in moment form: tau_g = 1");
+    // phi in 3 slots (collide_tile keeps 4; this pass does not read plane z's
+    // phi after the normals), so that with s_f two workgroups fit per CU.
+    __shared__ float s_phi[3][PN];
+    __shared__ float s_n[3][3][HN];
+    __shared__ float s_ind[2][NT];
+    __shared__ float s_f[SL::kSlots];
+
+    const Geo& G = c.geo;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const Derived d(c.phys);
+    const FmPlanes msrc(c.fsrc, (unsigned)G.plane), mdst(c.fdst, (unsigned)G.plane);
+    auto plane4 = [&](int z) { return (unsigned)(wrapmod(z, G.nz) * G.nxy) * 4u; };
+
+    // ---- phi ring and normals, as collide_tile (phi slot of plane z: (z + 3) mod 3, z >= -3)
+    auto pslot = [](int z) { return (z + 3) % 3; };
+    unsigned poff[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        const int e = tid + q * NT;
+        unsigned off = 0;
+        if (e < PN) {
+            const int lx = e % PX, ly = e / PX;
+            off = (unsigned)(wrapmod(y0 - 2 + ly, G.ny) * G.nx + wrapmod(x0 - 2 + lx, G.nx)) * 4u;
+        }
+        poff[q] = off;
+    }
+    const Field phi_f(c.phi, (unsigned)(G.n * 4));
+    auto load_phi = [&](int z) {
+        const unsigned z4 = plane4(z);
+        float* dst = s_phi[pslot(z)];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = phi_f.ld(poff[q], z4);
+        }
+    };
+    float preg[KP];
+    auto fetch_phi = [&](int z) {
+        const unsigned z4 = plane4(z);
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) preg[q] = phi_f.ld(poff[q], z4);
+        }
+    };
+    auto put_phi = [&](int z) {
+        float* dst = s_phi[pslot(z)];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            const int e = tid + q * NT;
+            if (q < KP - 1 || e < PN) dst[e] = preg[q];
+        }
+    };
+    auto nslot = [](int z) { return (z + 3) % 3; };  // z >= -3
+    auto comp_n = [&](int z) {
+        const float* pm = s_phi[pslot(z - 1)];
+        const float* p0 = s_phi[pslot(z)];
+        const float* pp = s_phi[pslot(z + 1)];
+        const int sl = nslot(z);
+        float* ind_dst = s_ind[z & 1];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) {
+            const int e = tid + q * NT;
+            if (q < KH - 1 || e < HN) {
+                const int hx = e % HX, hy = e / HX;
+                const int cc = (hy + 1) * PX + (hx + 1);
+                float nx, ny, nz, ind;
+                auto nb = [&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    return pick<CZ[l]>(pm, p0, pp)[cc + CY[l] * PX + CX[l]];
+                };
+                grad_normal(nb, d, nx, ny, nz, ind);
+                s_n[sl][0][e] = nx;
+                s_n[sl][1][e] = ny;
+                s_n[sl][2][e] = nz;
+                if (hx >= 1 && hx <= TX && hy >= 1 && hy <= TY) ind_dst[(hy - 1) * TX + (hx - 1)] = ind;
+            }
+        }
+    };
+
+    // ---- tile cell
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < G.nx && y < G.ny;
+    const unsigned own4 = (unsigned)(wrapmod(y, G.ny) * G.nx + wrapmod(x, G.nx)) * 4u;
+    const int row0 = ty * TX + tx, row1 = ty * (TX + 1) + tx;   // row bases of widths TX, TX + 1
+    auto own_slot = [&](auto lc) -> float& {
+        constexpr int l = decltype(lc)::value;
+        return s_f[(SL::w(l) == TX ? row0 : row1) + SL::own(l)];
+    };
+    auto load_m = [&](float (&v)[kFm], unsigned cell4, unsigned z4) {
+#pragma unroll
+        for (int k = 0; k < kFm; ++k) v[k] = msrc.ld(k, cell4, z4);
+    };
+    float hin[9];      // in-plane links (rest first, then c_z = 0 links in link order) for the next plane
+    float hup[2][5];   // c_z = +1 links by parity of their destination plane
+    // From a source plane's per-cell values: its c_z = -1 links written for
+    // the destination plane below, its in-plane links into hin and its c_z =
+    // +1 links into up.
+    auto tile_source = [&](const FluidCell& fc, bool write_down, float (&up)[5]) {
+        int ki = 0, ku = 0;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            const float v = fluid_post<l>(fc, c.phys, d);
+            if constexpr (CZ[l] < 0) {
+                if (write_down) own_slot(lc) = v;
+            } else if constexpr (CZ[l] == 0) {
+                hin[ki] = v;
+                ++ki;
+            } else {
+                up[ku] = v;
+                ++ku;
+            }
+        });
+    };
+    auto write_held = [&](const float (&up)[5]) {
+        int ki = 1, ku = 0;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (l != 0 && CZ[l] == 0) {
+                own_slot(lc) = hin[ki];
+                ++ki;
+            } else if constexpr (CZ[l] > 0) {
+                own_slot(lc) = up[ku];
+                ++ku;
+            }
+        });
+    };
+
+    // ---- ring cell (side 0: row y0-1, 1: row y0+TY, 2: column x0-1, 3: column x0+TX)
+    const bool ring = tid < NR;
+    int hx, hy, side;
+    if (tid < TX) { hx = tid + 1; hy = 0; side = 0; }
+    else if (tid < 2 * TX) { hx = tid - TX + 1; hy = TY + 1; side = 1; }
+    else if (tid < 2 * TX + TY + 2) { hx = 0; hy = tid - 2 * TX; side = 2; }
+    else { hx = TX + 1; hy = min(tid - 2 * TX - TY - 2, TY + 1); side = 3; }
+    const unsigned roff4 = (unsigned)(wrapmod(y0 - 1 + hy, G.ny) * G.nx + wrapmod(x0 - 1 + hx, G.nx)) * 4u;
+    int rslot[5];
+    {
+        const int sx = hx - 1, sy = hy - 1;
+        auto fill = [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            auto one = [&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int l = RING_LINK[s][k];
+                rslot[k] = SL::inside(l, sx, sy) ? SL::at(l, sx, sy) : SL::kSpare;
+            };
+            one(std::integral_constant<int, 0>{});
+            one(std::integral_constant<int, 1>{});
+            one(std::integral_constant<int, 2>{});
+            one(std::integral_constant<int, 3>{});
+            one(std::integral_constant<int, 4>{});
+        };
+        if (side == 0) fill(std::integral_constant<int, 0>{});
+        else if (side == 1) fill(std::integral_constant<int, 1>{});
+        else if (side == 2) fill(std::integral_constant<int, 2>{});
+        else fill(std::integral_constant<int, 3>{});
+    }
+    float rin[3];    // the ring cell's in-plane links for the next plane
+    float rup[2];    // its c_z = +1 link by parity of the destination plane
+    // The side's five links of a ring cell's plane: in-plane into rin, c_z =
+    // +1 into up, c_z = -1 returned.
+    auto ring_source = [&](const FluidCell& fc, float& up) {
+        float down = 0.0f;
+        auto by_side = [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            rin[0] = fluid_post<RING_LINK[s][0]>(fc, c.phys, d);
+            rin[1] = fluid_post<RING_LINK[s][1]>(fc, c.phys, d);
+            rin[2] = fluid_post<RING_LINK[s][2]>(fc, c.phys, d);
+            up = fluid_post<RING_LINK[s][3]>(fc, c.phys, d);
+            down = fluid_post<RING_LINK[s][4]>(fc, c.phys, d);
+        };
+        if (side == 0) by_side(std::integral_constant<int, 0>{});
+        else if (side == 1) by_side(std::integral_constant<int, 1>{});
+        else if (side == 2) by_side(std::integral_constant<int, 2>{});
+        else by_side(std::integral_constant<int, 3>{});
+        return down;
+    };
+
+    // Prologue: normals of planes zb-1 and zb (phi planes zb-2..zb+1 through
+    // the three slots); source planes zb-1 (c_z = +1 links for plane zb) and
+    // zb (in-plane links for plane zb, c_z = +1 links for plane zb+1).
+    load_phi(zb - 2);
+    load_phi(zb - 1);
+    load_phi(zb);
+    {
+        float v[kFm];
+        load_m(v, own4, plane4(zb - 1));
+        tile_source(fluid_cell_of(v), false, hup[0]);
+        load_m(v, own4, plane4(zb));
+        tile_source(fluid_cell_of(v), false, hup[1]);   // leaves plane zb's in-plane links in hin
+        if (ring) {
+            load_m(v, roff4, plane4(zb - 1));
+            ring_source(fluid_cell_of(v), rup[0]);
+            load_m(v, roff4, plane4(zb));
+            ring_source(fluid_cell_of(v), rup[1]);      // leaves plane zb's in-plane links in rin
+        }
+    }
+    __syncthreads();
+    comp_n(zb - 1);
+    __syncthreads();   // plane zb+1 replaces plane zb-2
+    load_phi(zb + 1);
+    fetch_phi(zb + 2);
+    __syncthreads();
+    comp_n(zb);
+    __syncthreads();   // plane zb+2 replaces plane zb-1 in the first iteration
+
+    auto iter = [&](int z, auto par) {
+        constexpr int PAR = decltype(par)::value;
+        float v[kFm], rv[kFm];
+        load_m(v, own4, plane4(z + 1));                 // used after the plane's barrier and normals
+        if (ring) load_m(rv, roff4, plane4(z + 1));
+        // The phi slot of z+2 held plane z-1, last read by comp_n(z) before
+        // the previous barrier; the n slot of z+1 held plane z-2 (read last
+        // in iteration z-1).
+        put_phi(z + 2);
+        __syncthreads();
+        comp_n(z + 1);
+        if (z + 1 < ze) fetch_phi(z + 3);
+        // Sources: plane z (held in-plane links), z-1 (held c_z = +1 links),
+        // z+1 (built now).
+        const float f0 = hin[0];
+        write_held(hup[PAR]);
+        tile_source(fluid_cell_of(v), true, hup[PAR]);
+        if (ring) {
+            s_f[rslot[0]] = rin[0];
+            s_f[rslot[1]] = rin[1];
+            s_f[rslot[2]] = rin[2];
+            s_f[rslot[3]] = rup[PAR];
+            s_f[rslot[4]] = ring_source(fluid_cell_of(rv), rup[PAR]);
+        }
+        __syncthreads();
+        if (!valid) return;
+        float fv[Q];
+        fv[0] = f0;
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            if constexpr (l != 0) fv[l] = s_f[(SL::w(l) == TX ? row0 : row1) + SL::src(l)];
+        });
+        const unsigned fb = (unsigned)(G.n * 4), z4 = (unsigned)(z * G.nxy) * 4u;
+        const int sm = nslot(z - 1), s0 = nslot(z), sp = nslot(z + 1);
+        const int e0 = (ty + 1) * HX + (tx + 1);
+        const float ncx = s_n[s0][0][e0], ncy = s_n[s0][1][e0], ncz = s_n[s0][2][e0];
+        const float ind = s_ind[z & 1][tid];
+        auto nbn = [&](auto lc, float& ax, float& ay, float& az) {
+            constexpr int l = decltype(lc)::value;
+            const int slot = pick<CZ[l]>(sm, s0, sp);
+            const int e = e0 + CY[l] * HX + CX[l];
+            if constexpr (CX[l] != 0) ax = s_n[slot][0][e];
+            if constexpr (CY[l] != 0) ay = s_n[slot][1][e];
+            if constexpr (CZ[l] != 0) az = s_n[slot][2][e];
+        };
+        float curv, Fx, Fy, Fz;
+        curvature_force(nbn, c.phys, d, ncx, ncy, ncz, ind, curv, Fx, Fy, Fz);
+        const Moments m = moments(fv, c.phys, Fx, Fy, Fz);
+        if constexpr (MODE == COLLIDE_MACROS) {
+            Field(c.rho, fb).st(own4, z4, m.rho);
+            Field(c.ux, fb).st(own4, z4, m.ux);
+            Field(c.uy, fb).st(own4, z4, m.uy);
+            Field(c.uz, fb).st(own4, z4, m.uz);
+            Field(c.fx, fb).st(own4, z4, Fx);
+            Field(c.fy, fb).st(own4, z4, Fy);
+            Field(c.fz, fb).st(own4, z4, Fz);
+            Field(c.nx, fb).st(own4, z4, ncx);
+            Field(c.ny, fb).st(own4, z4, ncy);
+            Field(c.nz, fb).st(own4, z4, ncz);
+        } else {
+            const float out[kFm] = {m.rho, m.ux, m.uy, m.uz, m.pxx, m.pyy, m.pzz, m.pxy, m.pxz, m.pyz, Fx, Fy, Fz};
+#pragma unroll
+            for (int k = 0; k < kFm; ++k) mdst.st(k, own4, z4, out[k]);
+            if (c.gm_n[0]) {
+                Field(c.gm_n[0], fb).st(own4, z4, ncx);
+                Field(c.gm_n[1], fb).st(own4, z4, ncy);
+                Field(c.gm_n[2], fb).st(own4, z4, ncz);
+            }
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    for (int z = zb; z < ze; z += 2) {
+        iter(z, I0{});
+        if (z + 1 < ze) iter(z + 1, I1{});
+    }
+}
+
 // ---------------------------------------------------------------- batched
 
 // One work item per workgroup: a TX x TY x zc block of one case. The host

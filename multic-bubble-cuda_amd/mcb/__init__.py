@@ -24,6 +24,7 @@ WEIGHTS_LITERAL, WEIGHTS_UNIT_SUM = 0, 1
 STORE_FP32, STORE_FP16, STORE_BF16 = 0, 1, 2
 PATH_AUTO, PATH_FUSED, PATH_STAGED = 0, 1, 2
 NORMALS_AUTO, NORMALS_STORED, NORMALS_RECOMPUTED = 0, 1, 2
+F_POPULATIONS, F_MOMENTS = 0, 1
 FIELDS = {"phi": 0, "rho": 1, "ux": 2, "uy": 3, "uz": 4, "fx": 5, "fy": 6, "fz": 7,
           "normx": 8, "normy": 9, "normz": 10, "f": 16, "g": 17}
 MAX_BUBBLES = 8
@@ -47,7 +48,7 @@ class Params(ctypes.Structure):
         ("path", ctypes.c_int), ("device", ctypes.c_int),
         ("tau_f", ctypes.c_float), ("tau_g", ctypes.c_float), ("sigma", ctypes.c_float),
         ("sharp_c", ctypes.c_float), ("moving_wall", ctypes.c_int),
-        ("normals", ctypes.c_int), ("reserved", ctypes.c_int * 6),
+        ("normals", ctypes.c_int), ("f_form", ctypes.c_int), ("reserved", ctypes.c_int * 5),
     ]
 
 
@@ -242,13 +243,14 @@ def reference_params():
 
 def make_params(nx, ny, nz, *, bc_mode=BC_PERIODIC, weights=WEIGHTS_UNIT_SUM, storage=STORE_FP32,
                 path=PATH_AUTO, device=0, tau_f=0.505, tau_g=1.0, sigma=0.1, sharp_c=-1.0, moving_wall=False,
-                normals=0):
+                normals=0, f_form=0):
     p = reference_params()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.bc_mode, p.weight_set, p.storage, p.path, p.device = bc_mode, weights, storage, path, device
     p.tau_f, p.tau_g, p.sigma, p.sharp_c = tau_f, tau_g, sigma, sharp_c
     p.moving_wall = int(moving_wall)
     p.normals = int(normals)
+    p.f_form = int(f_form)
     return p
 
 
@@ -296,7 +298,8 @@ class Lattice:
 
     @classmethod
     def from_case(cls, case, **overrides):
-        p = case.params
+        # A copy: the overrides apply to this lattice, not to the case.
+        p = Params.from_buffer_copy(case.params)
         for k, v in overrides.items():
             setattr(p, k, v)
         lat = cls(p)

@@ -51,7 +51,7 @@ def test_abi_version():
 @pytest.mark.parametrize("field,value", [
     ("nx", 2), ("ny", 0), ("nz", -4), ("tau_f", 0.5), ("tau_f", float("nan")), ("tau_g", 0.4),
     ("sigma", -1.0), ("sigma", float("inf")), ("bc_mode", 7), ("weight_set", 3), ("storage", 9),
-    ("path", 5), ("device", -1), ("normals", 3), ("normals", -1),
+    ("path", 5), ("device", -1), ("normals", 3), ("normals", -1), ("f_form", 2), ("f_form", -1),
 ])
 def test_validate_rejects(field, value):
     p = mcb.make_params(16, 16, 16)
@@ -104,6 +104,11 @@ def test_unsupported_combinations():
     for kw in (dict(tau_g=0.8), dict(moving_wall=True), dict(storage=mcb.STORE_FP16)):
         p = mcb.make_params(16, 16, 16, bc_mode=mcb.BC_REFERENCE, path=mcb.PATH_FUSED, **kw)
         assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED, kw
+    # f in moment form: fused FP32, g in moment form, periodic only.
+    for kw in (dict(tau_g=0.8), dict(storage=mcb.STORE_FP16), dict(path=mcb.PATH_STAGED),
+               dict(bc_mode=mcb.BC_REFERENCE)):
+        p = mcb.make_params(16, 16, 16, f_form=mcb.F_MOMENTS, **kw)
+        assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED, kw
 
 
 def test_footprint():
@@ -120,6 +125,10 @@ def test_footprint():
     assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) == moments - 3 * (n + 256) * 4
     p.normals = mcb.NORMALS_STORED
     assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) == moments
+    # f in moment form: 13 planes A/B instead of 19, u among them.
+    p.f_form = mcb.F_MOMENTS
+    assert mcb.lib().mcb_footprint_bytes(ctypes.byref(p)) == moments - (2 * 6 + 3) * (n + 256) * 4
+    p.f_form = mcb.F_POPULATIONS
     n2 = 256 ** 3
     p256 = mcb.make_params(256, 256, 256)
     auto = mcb.lib().mcb_footprint_bytes(ctypes.byref(p256))

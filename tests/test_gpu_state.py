@@ -205,6 +205,32 @@ def test_moment_form_images_cross_normal_modes(gpu_lib):
                 assert np.array_equal(bt.read(cid, name), ref[name]), f"batch from normals {src}: {name}"
 
 
+def test_f_moments_image_continues_bit_for_bit(gpu_lib):
+    """f in moment form: an image (the 13 moment planes and phi) resumes the
+    run bit for bit in a fresh context, in both normal modes; a context that
+    keeps f as populations refuses it."""
+    mcb = gpu_lib
+    p = mcb.make_params(*SHAPE, tau_f=0.6, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, f_form=mcb.F_MOMENTS)
+    with mcb.Lattice(p) as a:
+        a.init_bubbles(BUB, 3.0)
+        a.step(7)
+        image = a.save_state()
+        a.step(9)
+        ref = snapshot(a)
+    for normals in (mcb.NORMALS_STORED, mcb.NORMALS_RECOMPUTED):
+        p.normals = normals
+        b = mcb.Lattice(p)
+        b.load_state(image)
+        b.step(9)
+        assert_same(dict(ref), snapshot(b), f"f-moment image, normals {normals}")
+        b.close()
+    p.f_form = mcb.F_POPULATIONS
+    with mcb.Lattice(p) as c:
+        c.init_bubbles(BUB, 3.0)
+        with pytest.raises(mcb.McbError):
+            c.load_state(image)
+
+
 def test_state_images_of_other_cases_are_refused(gpu_lib):
     mcb = gpu_lib
     a = mcb.Lattice(params(mcb, mcb.STORE_FP32, 1.0))

@@ -44,16 +44,21 @@ def assert_parity(got, want, label, exact=True):
 # Fused FP32 at tau_g = 1 keeps g in moment form with the interface normal
 # either stored by the collide pass or recomputed by the phase pass.
 NORMALS = {"n-stored": 1, "n-recomputed": 2}
+# Tile shapes of the collide pass with f in moment form.
+FM_TILES = {(64, 8), (32, 8), (32, 16), (16, 16), (16, 8)}
 
 
-@pytest.mark.parametrize("path", ["fused", "fused-n-recomputed", "staged"])
+@pytest.mark.parametrize("path", ["fused", "fused-n-recomputed", "fused-f-moments", "staged"])
 @pytest.mark.parametrize("shape,tau_g", [((32, 32, 32), 1.0), ((48, 40, 36), 0.8), ((20, 24, 17), 1.0)])
 def test_periodic_short_bitexact(gpu_lib, path, shape, tau_g):
     mcb = gpu_lib
+    if path == "fused-f-moments" and tau_g != 1.0:
+        pytest.skip("f in moment form needs g in moment form (tau_g = 1)")
     nx, ny, nz = shape
     p = mcb.make_params(nx, ny, nz, tau_f=0.55, tau_g=tau_g, sigma=0.1,
                         path=mcb.PATH_STAGED if path == "staged" else mcb.PATH_FUSED,
-                        normals=mcb.NORMALS_RECOMPUTED if path == "fused-n-recomputed" else mcb.NORMALS_STORED)
+                        normals=mcb.NORMALS_RECOMPUTED if path == "fused-n-recomputed" else mcb.NORMALS_STORED,
+                        f_form=mcb.F_MOMENTS if path == "fused-f-moments" else mcb.F_POPULATIONS)
     bubbles = [(nx * 0.5 + 0.3, ny * 0.5 - 1.2, nz * 0.5 + 0.7, min(shape) * 0.25)]
     lat, o, got, want = run_pair(mcb, p, bubbles, 20)
     assert_parity(got, want, f"{path} {shape} tau_g={tau_g}")
@@ -82,16 +87,49 @@ def test_cfg0_1000_steps(gpu_lib, path):
     lat.close()
 
 
+@pytest.mark.parametrize("f_form", ["f-populations", "f-moments"])
 @pytest.mark.parametrize("normals", list(NORMALS))
-def test_moment_form_1000_steps(gpu_lib, normals):
+def test_moment_form_1000_steps(gpu_lib, normals, f_form):
     """tau_g = 1 keeps g as (phi, u, n) and recomputes g* in the phase pass,
-    with n stored or itself recomputed from phi: 1000 steps stay
-    bit-identical to the transcription."""
+    with n stored or itself recomputed from phi, and f as populations or as
+    the 13 moments collisionFluid reads: 1000 steps stay bit-identical to the
+    transcription."""
     mcb = gpu_lib
-    p = mcb.make_params(48, 48, 48, tau_f=0.55, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=NORMALS[normals])
+    p = mcb.make_params(48, 48, 48, tau_f=0.55, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=NORMALS[normals],
+                        f_form=mcb.F_MOMENTS if f_form == "f-moments" else mcb.F_POPULATIONS)
     bubbles = [(24.3, 23.6, 24.1, 10.0)]
     lat, o, got, want = run_pair(mcb, p, bubbles, 1000)
-    assert_parity(got, want, f"moment form ({normals}) 48^3 x 1000")
+    assert_parity(got, want, f"moment form ({normals}, {f_form}) 48^3 x 1000")
+    lat.close()
+
+
+def test_f_moments_read_populations_and_refuse_writes(gpu_lib):
+    """With f in moment form, f and g read in the reference layout equal the
+    transcription's populations before and after steps (f rebuilt from the
+    moments with collisionFluid's operations); populations cannot be
+    written in this form."""
+    mcb = gpu_lib
+    p = mcb.make_params(30, 22, 18, tau_f=0.6, tau_g=1.0, sigma=0.08, path=mcb.PATH_FUSED, f_form=mcb.F_MOMENTS)
+    bubbles = [(14.5, 10.0, 9.5, 5.5)]
+    o = oracle_for(p)
+    o.init(f32_bubbles(bubbles))
+    lat = mcb.Lattice(p)
+    lat.init_bubbles(bubbles)
+    for steps in (0, 1, 6):
+        lat.step(steps)
+        o.step(steps)
+        for name in ("f", "g"):
+            got = lat.read(name).reshape(19, -1)
+            assert np.array_equal(got, o.field(name).reshape(19, -1)), f"{name} after {steps} more steps"
+    with pytest.raises(mcb.McbError) as e:
+        lat.write("f", lat.read("f"))
+    assert e.value.status == mcb.ERR_UNSUPPORTED
+    with pytest.raises(mcb.McbError):
+        lat.write("g", lat.read("g"))
+    got = lat.observables()
+    o.macros()
+    for k in OBS:
+        assert np.array_equal(got[k], o.field(k)), k
     lat.close()
 
 
@@ -131,9 +169,13 @@ def test_fused_tilings_agree(gpu_lib, tau_g):
     mcb = gpu_lib
     bubbles = [(20.5, 17.0, 14.2, 7.5)]
     ref = None
-    for normals in (list(NORMALS.values()) if tau_g == 1.0 else [0]):
-        p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=tau_g, sigma=0.05, path=mcb.PATH_FUSED, normals=normals)
+    forms = [(n, mcb.F_POPULATIONS) for n in NORMALS.values()] + [(1, mcb.F_MOMENTS), (2, mcb.F_MOMENTS)]
+    for normals, f_form in (forms if tau_g == 1.0 else [(0, mcb.F_POPULATIONS)]):
+        p = mcb.make_params(40, 36, 28, tau_f=0.6, tau_g=tau_g, sigma=0.05, path=mcb.PATH_FUSED, normals=normals,
+                            f_form=f_form)
         for tx, ty in mcb.tilings():
+            if f_form == mcb.F_MOMENTS and (tx, ty) not in FM_TILES:
+                continue
             for zc in (1, 5, 28):
                 lat = mcb.Lattice(p)
                 lat.init_bubbles(bubbles)
@@ -144,7 +186,8 @@ def test_fused_tilings_agree(gpu_lib, tau_g):
                 if ref is None:
                     ref = obs
                 for k in OBS:
-                    assert np.array_equal(obs[k], ref[k]), f"normals {normals}, tiling {tx}x{ty} zc={zc}: {k} differs"
+                    assert np.array_equal(obs[k], ref[k]), \
+                        f"normals {normals}, f_form {f_form}, tiling {tx}x{ty} zc={zc}: {k} differs"
 
 
 def test_populations_roundtrip_and_layout(gpu_lib):
@@ -262,7 +305,7 @@ def test_reference_mode_fused_write_f_and_resume(gpu_lib, normals):
     lat.close()
 
 
-@pytest.mark.parametrize("kind", ["moments", "moments-n-recomputed", "populations", "reference"])
+@pytest.mark.parametrize("kind", ["moments", "moments-n-recomputed", "f-moments", "populations", "reference"])
 def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
     """mcb_step replays pairs of fused steps from a captured hipGraph. Each
     replay, with eager work (observables, a field readout) and host
@@ -276,6 +319,8 @@ def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
         kw.update(bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL)
     if kind == "moments-n-recomputed":
         kw["normals"] = mcb.NORMALS_RECOMPUTED
+    if kind == "f-moments":
+        kw["f_form"] = mcb.F_MOMENTS
     p = mcb.make_params(30, 26, 22, **kw)
     bubbles = [(15.2, 13.1, 11.3, 6.0)]
     g, e = mcb.Lattice(p), mcb.Lattice(p)
