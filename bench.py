@@ -155,6 +155,7 @@ def f_moments(mcb, case, steps, warmup):
             "b_moved_per_lu": B_ALG_COLLIDE_FM + B_ALG_PHASE_GM,
             "k_phase_ms": round(ph, 5), "k_collide_ms": round(co, 5),
             "k_collide_gbs": round(B_ALG_COLLIDE_FM * cells / (co * 1e-3) / 1e9, 1),
+            "k_collide_traffic": traffic_for("cfg2-256-fp32-fmoments"),
             "sum_phi": s, "finite": finite}
 
 
