@@ -24,6 +24,8 @@
 //               All arithmetic stays FP32. Each cell's rounding residual of
 //               links 1..18 is folded into its rest link (l = 0), so the
 //               cell's total is stored with one 16-bit rounding only.
+// FP32 with g in moment form can also keep f as the 13 values collisionFluid
+// reads (collide_tile_fm) instead of 19 populations.
 #pragma once
 #include "fused.hpp"
 #include "lattice_fast.hpp"
