@@ -441,15 +441,38 @@ std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
     }
     if (c->fm)   // the 13 moments (u among them) and the state's phi
         return {{"fm", c->f[c->cur], kFm, N, P, 4, false}, {"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false}};
-    std::vector<StatePart> parts{{"f", c->f[c->cur], 19, N, P, 4, false}};
+    // Images keep populations in planes: with blocked populations they pass
+    // through the idle A/B buffers (pops_to_image / pops_from_image).
+    const int side = c->geo.blocked ? c->cur ^ 1 : c->cur;
+    std::vector<StatePart> parts{{"f", c->f[side], 19, N, P, 4, false}};
     if (c->gm == GM_POPULATIONS) {
-        parts.push_back({"g", c->g[c->cur], 19, N, P, 4, false});
+        parts.push_back({"g", c->g[side], 19, N, P, 4, false});
     } else {
         static const char* const un[3] = {"u0", "u1", "u2"};
         parts.push_back({"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false});
         for (int k = 0; k < 3; ++k) parts.push_back({un[k], c->gmu[k], 1, N, N, 4, false});
     }
     return parts;
+}
+
+// Blocked populations around a state image (fused path): the current f (and
+// g populations) into the idle buffers in planes, or back. With ghost-layer
+// faces the idle f's face populations then go back to w_l (a step reads the
+// current buffer and writes every interior cell of the idle one).
+hipError_t pops_image_copy(mcb_ctx* c, bool to_image) {
+    const int cur = c->cur, idle = c->cur ^ 1;
+    float* const* bufs[2] = {c->f, c->gm == GM_POPULATIONS ? c->g : nullptr};
+    for (float* const* b : bufs) {
+        if (!b) continue;
+        const hipError_t e = to_image ? launch_link_shift(b[cur], 1, b[idle], 0, c->geo, 0, c->stream)
+                                      : launch_link_shift(b[idle], 0, b[cur], 1, c->geo, 0, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+hipError_t pops_image_done(mcb_ctx* c) {
+    if (c->p.bc_mode != MCB_BC_REFERENCE) return hipSuccess;
+    return launch_init_pull(c->phi, c->f[c->cur ^ 1], nullptr, c->geo, c->phys, c->stream);
 }
 
 float* scalar_field(mcb_ctx* c, int field) {
@@ -554,6 +577,8 @@ mcb_status mcb_create(const mcb_params* p, void* stream, mcb_ctx** out) {
         // (or n recomputed by the phase pass) instead of two sets of 19 planes.
         const bool moments = p->tau_g == 1.0f;
         c->fm = p->f_form == MCB_F_MOMENTS;
+        // Populations in 64-cell blocks where a z plane holds whole blocks.
+        c->geo.blocked = MCB_POP_BLOCKS && !c->fm && c->geo.nxy % 64 == 0;
         for (int b = 0; b < 2; ++b) {
             if ((st = alloc(c, &c->f[b], (c->fm ? kFm : 19) * (size_t)P)) != MCB_OK) return bail(st);
             if (!moments && (st = alloc(c, &c->g[b], 19 * (size_t)P)) != MCB_OK) return bail(st);
@@ -800,8 +825,8 @@ mcb_status mcb_read_field(mcb_ctx* c, int field, float* host, size_t n) {
                 MCB_HIP(c, launch_gm_to_ref(fused_view(c), idle_f, c->stream));
             } else {
                 const float* src = field == MCB_FIELD_F ? c->f[c->cur] : c->g[c->cur];
-                MCB_HIP(c, launch_link_shift(src, field == MCB_FIELD_F ? idle_f : c->g[c->cur ^ 1], c->geo, +1,
-                                             c->stream));
+                MCB_HIP(c, launch_link_shift(src, c->geo.blocked, field == MCB_FIELD_F ? idle_f : c->g[c->cur ^ 1],
+                                             0, c->geo, +1, c->stream));
             }
             const float* tmp = field == MCB_FIELD_F || c->gm != GM_POPULATIONS ? idle_f : c->g[c->cur ^ 1];
             for (int l = 0; l < 19; ++l)
@@ -858,7 +883,7 @@ mcb_status mcb_write_field(mcb_ctx* c, int field, const float* host, size_t n) {
             for (int l = 0; l < 19; ++l)
                 MCB_HIP(c, hipMemcpyAsync(tmp + (size_t)l * c->geo.plane, host + l * cells, cells * sizeof(float),
                                           hipMemcpyHostToDevice, c->stream));
-            MCB_HIP(c, launch_link_shift(tmp, dst, c->geo, -1, c->stream));
+            MCB_HIP(c, launch_link_shift(tmp, 0, dst, c->geo.blocked, c->geo, -1, c->stream));
             if (c->p.bc_mode == MCB_BC_REFERENCE) {
                 // The written face populations feed the next step once (as
                 // the reference's first momentiCalc reads them); after it they
@@ -1163,7 +1188,12 @@ mcb_status mcb_save_state(mcb_ctx* c, void* buf, size_t bytes) {
     h.path = c->path;
     h.gm = c->path == MCB_PATH_STAGED ? GM_POPULATIONS : c->gm;
     h.steps = c->steps;
-    return state_image_save(c->stream, h, ctx_state_parts(c), buf, bytes, &c->err);
+    const bool staged = c->path == MCB_PATH_FUSED && c->geo.blocked;
+    if (staged) MCB_HIP(c, pops_image_copy(c, true));
+    st = state_image_save(c->stream, h, ctx_state_parts(c), buf, bytes, &c->err);
+    if (staged) MCB_HIP(c, pops_image_done(c));
+    MCB_HIP(c, hipStreamSynchronize(c->stream));
+    return st;
 }
 
 mcb_status mcb_load_state(mcb_ctx* c, const void* buf, size_t bytes) {
@@ -1196,6 +1226,11 @@ mcb_status mcb_load_state(mcb_ctx* c, const void* buf, size_t bytes) {
         c->ref_reset_f = -1;   // the image's face populations are w_l
     }
     if ((st = state_image_load(c->stream, buf, bytes, ctx_state_parts(c), &c->err)) != MCB_OK) return st;
+    if (c->path == MCB_PATH_FUSED && c->geo.blocked) {
+        MCB_HIP(c, pops_image_copy(c, false));
+        MCB_HIP(c, pops_image_done(c));
+        MCB_HIP(c, hipStreamSynchronize(c->stream));
+    }
     if (c->path == MCB_PATH_FUSED && c->gm != GM_POPULATIONS) {
         c->phi = c->phis[c->cur ^ 1];
         if (c->gmn[0]) {   // n is not in the image: gradCalc of the state's phi

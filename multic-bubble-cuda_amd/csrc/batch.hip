@@ -285,6 +285,8 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
     c->geo.nxy = p.nx * p.ny;
     c->geo.n = (long long)p.nx * p.ny * p.nz;
     c->geo.plane = P;
+    // FP32 populations in 64-cell blocks where a z plane holds whole blocks.
+    c->geo.blocked = MCB_POP_BLOCKS && p.storage == MCB_STORE_FP32 && c->geo.nxy % 64 == 0;
     c->phys = host_phys(p);
     auto alloc = [&](size_t bytes, void** out) -> mcb_status {
         mcb_status s = dev_alloc(bytes, out);
@@ -621,7 +623,8 @@ mcb_status Batch::read(int id, int field, float* host, size_t n) {
             e = c->storage == MCB_STORE_FP32 ? launch_gm_to_ref(c->view(c->cur), tmp, stream_)
                                              : launch_ab_to_ref(c->view(c->cur), tmp, stream_);
         else if (c->storage == MCB_STORE_FP32)
-            e = launch_link_shift(static_cast<const float*>(isf ? c->f[c->cur] : c->g[c->cur]), tmp, c->geo, +1, stream_);
+            e = launch_link_shift(static_cast<const float*>(isf ? c->f[c->cur] : c->g[c->cur]), c->geo.blocked, tmp, 0,
+                                  c->geo, +1, stream_);
         else
             e = launch_decode16(isf ? c->f[c->cur] : c->g[c->cur], isf ? nullptr : c->phi[c->cur ^ 1], tmp, c->geo,
                                 c->phys, c->storage, stream_);
@@ -673,16 +676,22 @@ mcb_status Batch::to_populations(CaseState* c) {
 
 // What the next step of a case reads (view(cur)): f of the current buffer;
 // g as populations, or its moment form (u in FP32; B in 16 bits); the
-// state's phi (the moment form's phi, the 16-bit g anchor); the 16-bit mass
-// target and the correction pending for the next phase pass. n is not part
-// of an image: it is gradCalc of the state's phi, rebuilt on loading.
+// state's phi where a step reads it (the moment form's phi, the 16-bit g
+// anchor; FP32 g populations alone give phi); the 16-bit mass target and the
+// correction pending for the next phase pass. n is not part of an image: it
+// is gradCalc of the state's phi, rebuilt on loading. FP32 images have the
+// parts of a single context's (same form), so either resumes in the other.
 std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     const unsigned eb = pop_bytes(c->storage);
+    // Images keep populations in planes: blocked ones pass through the idle
+    // A/B buffers (stage_pops), which the next step overwrites.
+    const int side = c->geo.blocked ? c->cur ^ 1 : c->cur;
     std::vector<StatePart> parts;
-    parts.push_back({"f", c->f[c->cur], 19, N, P, eb, false});
-    if (c->gm == GM_POPULATIONS) parts.push_back({"g", c->g[c->cur], 19, N, P, eb, false});
-    parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
+    parts.push_back({"f", c->f[side], 19, N, P, eb, false});
+    if (c->gm == GM_POPULATIONS) parts.push_back({"g", c->g[side], 19, N, P, eb, false});
+    if (c->storage != MCB_STORE_FP32 || c->gm != GM_POPULATIONS)
+        parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
     if (c->gm != GM_POPULATIONS) {
         static const char* const un[3] = {"u0", "u1", "u2"};
         for (int k = 0; k < 3; ++k)
@@ -725,7 +734,23 @@ mcb_status Batch::save_state(int id, void* buf, size_t bytes) {
     h.path = MCB_PATH_FUSED;
     h.gm = c->gm;
     h.steps = c->steps;
+    if (c->geo.blocked) B_HIP(stage_pops(c, true));
     return state_image_save(stream_, h, state_parts(c, m5), buf, bytes, &err_);
+}
+
+// Blocked FP32 populations around a state image: the current f (and g
+// populations) into the idle buffers in planes, or back.
+hipError_t Batch::stage_pops(CaseState* c, bool to_image) {
+    const int cur = c->cur, idle = c->cur ^ 1;
+    void* const* bufs[2] = {c->f, c->gm == GM_POPULATIONS ? c->g : nullptr};
+    for (void* const* b : bufs) {
+        if (!b) continue;
+        const float* src = static_cast<const float*>(b[to_image ? cur : idle]);
+        float* dst = static_cast<float*>(b[to_image ? idle : cur]);
+        const hipError_t e = launch_link_shift(src, to_image ? 1 : 0, dst, to_image ? 0 : 1, c->geo, 0, stream_);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
@@ -746,6 +771,10 @@ mcb_status Batch::load_state(int id, const void* buf, size_t bytes) {
     c->gm = h.gm;
     double m5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if ((st = state_image_load(stream_, buf, bytes, state_parts(c, m5), &err_)) != MCB_OK) return st;
+    if (c->geo.blocked) {
+        B_HIP(stage_pops(c, false));
+        B_HIP(hipStreamSynchronize(stream_));
+    }
     if (c->mass) {
         MassCell m{m5[0], {0.0, 0.0}};
         m.delta[c->cur] = m5[1];
@@ -818,7 +847,8 @@ mcb_status Batch::write(int id, int field, const float* host, size_t n) {
             e = hipMemcpyAsync(tmp + l * P, host + l * cells, cells * sizeof(float), hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess) {
             if (c->storage == MCB_STORE_FP32)
-                e = launch_link_shift(tmp, static_cast<float*>(isf ? c->f[c->cur] : c->g[c->cur]), c->geo, -1, stream_);
+                e = launch_link_shift(tmp, 0, static_cast<float*>(isf ? c->f[c->cur] : c->g[c->cur]), c->geo.blocked,
+                                      c->geo, -1, stream_);
             else
                 e = launch_encode16(tmp, isf ? nullptr : c->phi[c->cur ^ 1], isf ? c->f[c->cur] : c->g[c->cur], c->geo,
                                     c->phys, c->storage, stream_);

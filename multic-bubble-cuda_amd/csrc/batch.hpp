@@ -125,6 +125,7 @@ private:
     // of the moment form (2: fixed-point target, 3: target recorded, 4:
     // pending correction).
     std::vector<StatePart> state_parts(CaseState* c, double* mass);
+    hipError_t stage_pops(CaseState* c, bool to_image);
     // One set of macro fields per batch, sized for its largest case; it holds
     // the observables of the case of the most recent macros() call.
     mcb_status ensure_macro_scratch();

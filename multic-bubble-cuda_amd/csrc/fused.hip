@@ -41,9 +41,9 @@ __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, un
                             (unsigned)(shift<1>(y, G.ny) * G.nx)};
     const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
                             (unsigned)(shift<1>(z, G.nz) * G.nxy)};
-    const PopPlanes<StoreF32> gp(c.gsrc, (unsigned)G.plane);
+    const PopPlanes<StoreF32> gp(c.gsrc, G);
     const unsigned me = zs[1] + ys[1] + xs[1];
-    float v = phase_sum<StoreF32>(c, gp, xs, ys, zs);
+    float v = phase_sum<StoreF32>(c, gp, Columns(xs, ys, gp), xs, ys, zs);
     // Ghost-layer faces (moment form): before the first step a face keeps its
     // initial phi, later it takes boundaryConditions' copy.
     if (c.bc == MCB_BC_REFERENCE && ref_face(G, x, y, z))
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(PBX * PBY) void k_init_pull(const float* __restrict
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const float w = p.w[WC[l]];
-        f[(size_t)l * G.plane + me] = w * 1.0f;
-        if (g) g[(size_t)l * G.plane + me] = w * phi[zs[1 + CZ[l]] + ys[1 + CY[l]] + xs[1 + CX[l]]];
+        f[pop_index(G, G.blocked, l, me)] = w * 1.0f;
+        if (g) g[pop_index(G, G.blocked, l, me)] = w * phi[zs[1 + CZ[l]] + ys[1 + CY[l]] + xs[1 + CX[l]]];
     });
 }
 
@@ -198,9 +198,11 @@ __global__ __launch_bounds__(PBX * PBY) void k_fm_to_ref(CaseView c, float* __re
     });
 }
 
+// dst_l(x) = src_l(x - SIGN c_l), src in layout sblk and dst in dblk (SIGN 0:
+// a change of layout).
 template <int SIGN>
-__global__ __launch_bounds__(PBX * PBY) void k_link_shift(const float* __restrict__ src, float* __restrict__ dst,
-                                                          Geo G) {
+__global__ __launch_bounds__(PBX * PBY) void k_link_shift(const float* __restrict__ src, int sblk,
+                                                          float* __restrict__ dst, int dblk, Geo G) {
     const int x = blockIdx.x * PBX + threadIdx.x;
     const int y = blockIdx.y * PBY + threadIdx.y;
     const int z = blockIdx.z;
@@ -210,8 +212,8 @@ __global__ __launch_bounds__(PBX * PBY) void k_link_shift(const float* __restric
     const int zs[3] = {shift<-1>(z, G.nz) * G.nxy, z * G.nxy, shift<1>(z, G.nz) * G.nxy};
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        const size_t pl = (size_t)l * G.plane;
-        dst[pl + zs[1] + ys[1] + xs[1]] = src[pl + zs[1 - SIGN * CZ[l]] + ys[1 - SIGN * CY[l]] + xs[1 - SIGN * CX[l]]];
+        dst[pop_index(G, dblk, l, zs[1] + ys[1] + xs[1])] =
+            src[pop_index(G, sblk, l, zs[1 - SIGN * CZ[l]] + ys[1 - SIGN * CY[l]] + xs[1 - SIGN * CX[l]])];
     });
 }
 
@@ -492,9 +494,12 @@ hipError_t launch_gm_to_ref(const CaseView& c, float* dst, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int sign, hipStream_t s) {
-    if (sign > 0) hipLaunchKernelGGL(k_link_shift<1>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, src, dst, geo);
-    else hipLaunchKernelGGL(k_link_shift<-1>, cell_grid(geo), dim3(PBX, PBY, 1), 0, s, src, dst, geo);
+hipError_t launch_link_shift(const float* src, int sblk, float* dst, int dblk, const Geo& geo, int sign,
+                             hipStream_t s) {
+    const dim3 grid = cell_grid(geo), block(PBX, PBY, 1);
+    if (sign > 0) hipLaunchKernelGGL(k_link_shift<1>, grid, block, 0, s, src, sblk, dst, dblk, geo);
+    else if (sign < 0) hipLaunchKernelGGL(k_link_shift<-1>, grid, block, 0, s, src, sblk, dst, dblk, geo);
+    else hipLaunchKernelGGL(k_link_shift<0>, grid, block, 0, s, src, sblk, dst, dblk, geo);
     return hipGetLastError();
 }
 

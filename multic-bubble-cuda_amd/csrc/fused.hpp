@@ -163,7 +163,8 @@ bool phase_recomputes_n(const Geo& geo, const Tiling& t);
 hipError_t launch_normals(const CaseView& c, hipStream_t s);
 hipError_t launch_collide(const CaseView& c, int mode, const Tiling& t, hipStream_t s);
 // f*_l(y) = w_l * 1, g*_l(y) = w_l * phi(y + c_l): the pull-form image of
-// initTensor + initDist (src/init.cu:21-24, 69-79). g may be null (moment form).
+// initTensor + initDist (src/init.cu:21-24, 69-79), in geo's population
+// layout. g may be null (moment form).
 hipError_t launch_init_pull(const float* phi, float* f, float* g, const Geo& geo, const Phys& p, hipStream_t s);
 // Moment form of g -> reference layout (post-stream g_l(x), 19 planes):
 // FP32 (phi, u) form (n recomputed), and the (phi, B) form of the 16-bit storage forms.
@@ -176,9 +177,12 @@ hipError_t launch_fm_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // Tile shapes instantiated for the f-moment collide pass.
 bool fm_tiling_supported(const Tiling& t);
 hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s);
-// dst_l(x) = src_l(x - sign*c_l) for all 19 planes (sign = +1: pull form ->
-// reference layout; sign = -1: reference layout -> pull form).
-hipError_t launch_link_shift(const float* src, float* dst, const Geo& geo, int sign, hipStream_t s);
+// dst_l(x) = src_l(x - sign*c_l) for all 19 links (sign = +1: pull form ->
+// reference order; sign = -1: reference order -> pull form; 0: a copy), src
+// in population layout sblk and dst in dblk (Geo::blocked values; the
+// reference order is always in planes).
+hipError_t launch_link_shift(const float* src, int sblk, float* dst, int dblk, const Geo& geo, int sign,
+                             hipStream_t s);
 
 // ---- batched (many cases, one launch per pass), any storage form
 // storage: MCB_STORE_FP32 / FP16 / BF16; work: device array of n_work items.

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_init16(const float* __restrict__ 
     if (x >= G.nx || y >= G.ny) return;
     const Offs o(x, y, z, G);
     const unsigned me = o.me();
-    const PopPlanes<S> fp(f, (unsigned)G.plane), gp(g, (unsigned)G.plane);
+    const PopPlanes<S> fp(f, G), gp(g, G);
     for_links([&](auto lc) { fp.template st<decltype(lc)::value>(me, 0.0f); });
     if (!g) return;
     const float ph = phi[me];
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_decode16(const void* src, const f
     const int z = blockIdx.z;
     if (x >= G.nx || y >= G.ny) return;
     const Offs o(x, y, z, G);
-    const PopPlanes<S> sp(src, (unsigned)G.plane);
+    const PopPlanes<S> sp(src, G);
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         const unsigned s = o.template nb<l, -1>();
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_encode16(const float* __restrict_
     if (x >= G.nx || y >= G.ny) return;
     const Offs o(x, y, z, G);
     const unsigned me = o.me();
-    const PopPlanes<S> dp(dst, (unsigned)G.plane);
+    const PopPlanes<S> dp(dst, G);
     const float a = anchor ? anchor[me] : 1.0f;
     float r = 0.0f;
     for_links([&](auto lc) {

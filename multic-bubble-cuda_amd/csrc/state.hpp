@@ -14,7 +14,7 @@
 
 namespace mcb {
 
-constexpr uint32_t kStateVersion = 2;
+constexpr uint32_t kStateVersion = 3;
 
 struct StateHeader {
     char magic[8];          // "MCBSTATE"

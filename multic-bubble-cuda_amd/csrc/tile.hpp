@@ -107,23 +107,25 @@ struct StoreBF16 {
     static __device__ __forceinline__ unsigned short enc(float v) { return __builtin_bit_cast(unsigned short, (__bf16)v); }
 };
 
-// 19 population planes of one buffer behind one buffer resource (see Planes
-// in common.hpp); 16-bit forms move raw offsets.
+// The 19 populations of one buffer behind one buffer resource (see Planes
+// in common.hpp); 16-bit forms move raw offsets and are always kept in
+// planes.
 template <class S> struct PopPlanes;
 template <> struct PopPlanes<StoreF32> : Planes {
     using Raw = float;   // a population as loaded (see raw())
-    __device__ __forceinline__ PopPlanes(const void* base, unsigned plane_elems)
-        : Planes(static_cast<const float*>(base), plane_elems) {}
+    __device__ __forceinline__ PopPlanes(const void* base, const Geo& G) : Planes(static_cast<const float*>(base), G) {}
     template <int L> __device__ __forceinline__ Raw raw(unsigned vb, unsigned sb) const { return ldb<L>(vb, sb); }
     static __device__ __forceinline__ float dec(Raw v) { return v; }
 };
 template <class S> struct PopPlanes {
     __amdgpu_buffer_rsrc_t r;
     unsigned plane_bytes;
-    __device__ __forceinline__ PopPlanes(const void* base, unsigned plane_elems)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(19u * plane_elems * 2u),
+    __device__ __forceinline__ PopPlanes(const void* base, const Geo& G)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(19u * (unsigned)G.plane * 2u),
                                               kBufferConfigWord)),
-          plane_bytes(plane_elems * 2u) {}
+          plane_bytes((unsigned)G.plane * 2u) {}
+    __device__ __forceinline__ unsigned col(unsigned e) const { return e << 1; }
+    __device__ __forceinline__ unsigned zb(unsigned zc) const { return zc << 1; }
     template <int L> __device__ __forceinline__ float ld(unsigned elem) const {
         return S::dec(__builtin_amdgcn_raw_buffer_load_b16(r, elem << 1, L * plane_bytes, 0));
     }
@@ -174,17 +176,19 @@ struct Field16 {
     }
 };
 
-// Byte offsets within a plane of the 9 columns (x - c_x, y - c_y) that a
-// cell's pull-form gathers read, for elements of EB bytes. They do not depend
-// on z, so a tile computes them once; per plane only the wave-uniform plane
-// offset changes, and it goes into the scalar offset of the access.
-template <unsigned EB> struct Columns {
+// In-plane byte offsets of the 9 columns (x - c_x, y - c_y) that a cell's
+// pull-form gathers read, in the layout of the population buffers (planes'
+// col). They do not depend on z, so a tile computes them once; per plane only
+// the wave-uniform plane offset (zb) changes, and it goes into the scalar
+// offset of the access.
+struct Columns {
     unsigned b[9];
-    __device__ __forceinline__ Columns(const unsigned (&xs)[3], const unsigned (&ys)[3]) {
+    template <class P>
+    __device__ __forceinline__ Columns(const unsigned (&xs)[3], const unsigned (&ys)[3], const P& planes) {
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) b[j * 3 + i] = (ys[j] + xs[i]) * EB;
+            for (int i = 0; i < 3; ++i) b[j * 3 + i] = planes.col(ys[j] + xs[i]);
     }
     template <int L> __device__ __forceinline__ unsigned src() const { return b[(1 - CY[L]) * 3 + (1 - CX[L])]; }
     __device__ __forceinline__ unsigned me() const { return b[4]; }
@@ -243,9 +247,11 @@ template <int L> __device__ __forceinline__ float g_from_moments(const CaseView&
 // From stored populations, or before the first step from initDist's
 // g_l = w_l phi0 (GM_INITIAL); the moment form runs phase_tile_gm (FP32) or
 // phase_tile_ab (16-bit forms) instead.
+// col: the columns of xs/ys in gp's layout.
 template <class S>
-__device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const unsigned (&xs)[3],
-                                           const unsigned (&ys)[3], const unsigned (&zs)[3]) {
+__device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>& gp, const Columns& col,
+                                           const unsigned (&xs)[3], const unsigned (&ys)[3],
+                                           const unsigned (&zs)[3]) {
     float sum = 0.0f;
     if (c.gm == GM_INITIAL) {
         const float p0 = ld32(c.phi_prev, zs[1] + ys[1] + xs[1]);
@@ -254,9 +260,10 @@ __device__ __forceinline__ float phase_sum(const CaseView& c, const PopPlanes<S>
     }
     for_links([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        const unsigned src = zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]];
-        if constexpr (S::kAnchored) sum += pop_from_offset<l>(c.phys, ld32(c.phi_prev, src), gp.template ld<l>(src));
-        else sum += gp.template ld<l>(src);
+        const float v = gp.template ldb<l>(col.template src<l>(), gp.zb(zs[1 - CZ[l]]));
+        if constexpr (S::kAnchored)
+            sum += pop_from_offset<l>(c.phys, ld32(c.phi_prev, zs[1 - CZ[l]] + ys[1 - CY[l]] + xs[1 - CX[l]]), v);
+        else sum += v;
     });
     return sum;
 }
@@ -1027,7 +1034,7 @@ __device__ __forceinline__ void phase_tile_ab(const CaseView& c, int x0, int y0,
 template <int MODE, class S, class NBN>
 __device__ __forceinline__ void cell16(const CaseView& c, const fast::Consts& k, const PopPlanes<S>& fsrc,
                                        const PopPlanes<S>& gsrc, const PopPlanes<S>& fdst,
-                                       const PopPlanes<S>& gdst, const Columns<2>& col, unsigned me4,
+                                       const PopPlanes<S>& gdst, const Columns& col, unsigned me4,
                                        const unsigned (&zs)[3], const unsigned short (&fr)[Q], NBN&& nbn,
                                        float ncx, float ncy, float ncz, float ind, float phic) {
     float Fx, Fy, Fz;
@@ -1124,8 +1131,8 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const Derived d(c.phys);
     const fast::Consts k16(c.phys, off_scale(1), off_scale(2));
-    const PopPlanes<S> fsrc(c.fsrc, (unsigned)G.plane), fdst(c.fdst, (unsigned)G.plane);
-    const PopPlanes<S> gsrc(c.gsrc, (unsigned)G.plane), gdst(c.gdst, (unsigned)G.plane);
+    const PopPlanes<S> fsrc(c.fsrc, G), fdst(c.fdst, G);
+    const PopPlanes<S> gsrc(c.gsrc, G), gdst(c.gdst, G);
 
     // In-plane byte offsets of the halo phi elements this thread stages.
     unsigned poff[KP];
@@ -1212,7 +1219,7 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                             valid ? (unsigned)shift<1>(x, G.nx) : 0u};
     const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
                             valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
-    const Columns<S::kBytes> col(xs, ys);
+    const Columns col(xs, ys, fsrc);   // f and g share the layout
     const unsigned me4 = (ys[1] + xs[1]) * 4u;   // FP32 fields
 
     // Prologue: phi planes zb-2..zb+1, normals of planes zb-1 and zb.
@@ -1235,7 +1242,7 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                                 (unsigned)(shift<1>(z, G.nz) * G.nxy)};
         for_links([&](auto lc) {
             constexpr int l = decltype(lc)::value;
-            r[l] = fsrc.template raw<l>(col.template src<l>(), zs[1 - CZ[l]] * S::kBytes);
+            r[l] = fsrc.template raw<l>(col.template src<l>(), fsrc.zb(zs[1 - CZ[l]]));
         });
     };
     // Plane z of the tile, given its gathered populations fr.
@@ -1310,12 +1317,12 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                 if constexpr (MODE == COLLIDE_STEP_BGK) {
                     for_links([&](auto lc) {
                         constexpr int l = decltype(lc)::value;
-                        gpre[l] = gsrc.template ldb<l>(col.template src<l>(), zs[1 - CZ[l]] * 4u);
+                        gpre[l] = gsrc.template ldb<l>(col.template src<l>(), gsrc.zb(zs[1 - CZ[l]]));
                     });
                 }
                 for_links([&](auto lc) {
                     constexpr int l = decltype(lc)::value;
-                    fdst.template stb<l>(col.me(), z1 * 4u, fluid_post<l>(fc, c.phys, d));
+                    fdst.template stb<l>(col.me(), fdst.zb(z1), fluid_post<l>(fc, c.phys, d));
                 });
                 if (MODE == COLLIDE_STEP && c.gm != GM_POPULATIONS) {
                     // Moment form: keep what collisionPhase needs (phic is the
@@ -1334,9 +1341,9 @@ __device__ __forceinline__ void collide_tile(const CaseView& c, int x0, int y0, 
                 for_links([&](auto lc) {
                     constexpr int l = decltype(lc)::value;
                     if constexpr (MODE == COLLIDE_STEP) {
-                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post<l>(pc, c.phys));
+                        gdst.template stb<l>(col.me(), gdst.zb(z1), phase_post<l>(pc, c.phys));
                     } else {
-                        gdst.template stb<l>(col.me(), z1 * 4u, phase_post_bgk<l>(pc, c.phys, d, gpre[l]));
+                        gdst.template stb<l>(col.me(), gdst.zb(z1), phase_post_bgk<l>(pc, c.phys, d, gpre[l]));
                     }
                 });
             }
@@ -1758,7 +1765,8 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
                             valid ? (unsigned)shift<1>(x, G.nx) : 0u};
     const unsigned ys[3] = {valid ? (unsigned)(shift<-1>(y, G.ny) * G.nx) : 0u, (unsigned)(y * G.nx),
                             valid ? (unsigned)(shift<1>(y, G.ny) * G.nx) : 0u};
-    const PopPlanes<S> gp(c.gsrc, (unsigned)G.plane);
+    const PopPlanes<S> gp(c.gsrc, G);
+    const Columns col(xs, ys, gp);
     const int ze = min(w.z0 + c.zc, G.nz);
     double part = 0.0;
     float delta = 0.0f;
@@ -1767,7 +1775,7 @@ __global__ __launch_bounds__(TX * TY) void k_phase_batch(const CaseView* __restr
         for (int z = w.z0; z < ze; ++z) {
             const unsigned zs[3] = {(unsigned)(shift<-1>(z, G.nz) * G.nxy), (unsigned)(z * G.nxy),
                                     (unsigned)(shift<1>(z, G.nz) * G.nxy)};
-            float v = phase_sum<S>(c, gp, xs, ys, zs);
+            float v = phase_sum<S>(c, gp, col, xs, ys, zs);
             if (mass) {
                 v += delta;
                 part += (double)v;

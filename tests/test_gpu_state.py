@@ -16,15 +16,19 @@ pytestmark = pytest.mark.gpu
 
 BUB = [(11.3, 9.6, 8.2, 5.0), (3.1, 14.2, 12.5, 3.5)]
 SHAPE = (24, 20, 18)            # nx, ny, nz
+# FP32 contexts keep populations in planes at SHAPE (nx*ny = 480) and in
+# 64-cell blocks where nx*ny is a multiple of 64 (the second shape); images
+# hold planes either way.
+SHAPES = [SHAPE, (32, 24, 18)]
 
 
-def params(mcb, storage, tau_g, path=None, bc=None):
+def params(mcb, storage, tau_g, path=None, bc=None, shape=SHAPE):
     kw = dict(tau_f=0.6, tau_g=tau_g, sigma=0.1, storage=storage)
     if path is not None:
         kw["path"] = path
     if bc is not None:
         kw["bc_mode"] = bc
-    return mcb.make_params(*SHAPE, **kw)
+    return mcb.make_params(*shape, **kw)
 
 
 def snapshot(lat):
@@ -44,18 +48,19 @@ FORMS = [
 ]
 
 
+@pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("k", [0, 7])
 @pytest.mark.parametrize("form", FORMS, ids=lambda f: f"{f[0]}-taug{f[1]}")
-def test_save_load_continues_bit_for_bit(gpu_lib, form, k):
+def test_save_load_continues_bit_for_bit(gpu_lib, form, k, shape):
     mcb = gpu_lib
     st = {"fp32": mcb.STORE_FP32, "fp16": mcb.STORE_FP16, "bf16": mcb.STORE_BF16}[form[0]]
-    p = params(mcb, st, form[1])
+    p = params(mcb, st, form[1], shape=shape)
     a = mcb.Lattice(p)
     a.init_bubbles(BUB, 3.0)
     a.step(k)
     image = a.save_state()
     pi, steps = mcb.state_info(image)
-    assert steps == k and (pi.nx, pi.ny, pi.nz) == SHAPE
+    assert steps == k and (pi.nx, pi.ny, pi.nz) == shape
     a.step(9)
     ref = snapshot(a)
     a.close()
@@ -76,10 +81,11 @@ def test_save_load_continues_bit_for_bit(gpu_lib, form, k):
     c.close()
 
 
-def test_fp32_state_matches_the_oracle_after_restart(gpu_lib):
+@pytest.mark.parametrize("shape", SHAPES)
+def test_fp32_state_matches_the_oracle_after_restart(gpu_lib, shape):
     """The resumed FP32 run equals the FP32 transcription run straight through."""
     mcb = gpu_lib
-    p = params(mcb, mcb.STORE_FP32, 1.0)
+    p = params(mcb, mcb.STORE_FP32, 1.0, shape=shape)
     a = mcb.Lattice(p)
     a.init_bubbles(BUB, 3.0)
     a.step(5)
@@ -99,13 +105,14 @@ def test_fp32_state_matches_the_oracle_after_restart(gpu_lib):
     o.close()
 
 
+@pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
-def test_populations_image_into_moment_form_context(gpu_lib, storage):
+def test_populations_image_into_moment_form_context(gpu_lib, storage, shape):
     """At tau_g = 1 writing g switches a case to stored g populations; its
     image loads into a fresh (moment-form) context, which switches too."""
     mcb = gpu_lib
     st = mcb.STORE_FP32 if storage == "fp32" else mcb.STORE_FP16
-    p = params(mcb, st, 1.0)
+    p = params(mcb, st, 1.0, shape=shape)
     a = mcb.Lattice(p)
     a.init_bubbles(BUB, 3.0)
     a.step(4)
@@ -171,7 +178,8 @@ def test_batch_case_resumes_in_another_batch(gpu_lib, storage):
         b2.step(2)
 
 
-def test_moment_form_images_cross_normal_modes(gpu_lib):
+@pytest.mark.parametrize("shape", SHAPES)
+def test_moment_form_images_cross_normal_modes(gpu_lib, shape):
     """An FP32 moment-form image holds phi and u, not the normal n (gradCalc
     of that phi): a context that stores n rebuilds it on loading, one that
     recomputes n needs nothing, and a batch case (n stored) resumes from a
@@ -181,7 +189,7 @@ def test_moment_form_images_cross_normal_modes(gpu_lib):
     modes = (mcb.NORMALS_STORED, mcb.NORMALS_RECOMPUTED)
     ref = None
     for src in modes:
-        p = mcb.make_params(*SHAPE, tau_f=0.6, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=src)
+        p = mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=src)
         with mcb.Lattice(p) as a:
             a.init_bubbles(BUB, 3.0)
             a.step(7)
@@ -203,6 +211,36 @@ def test_moment_form_images_cross_normal_modes(gpu_lib):
             bt.step(9)
             for name in ("f", "g"):
                 assert np.array_equal(bt.read(cid, name), ref[name]), f"batch from normals {src}: {name}"
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("tau_g", [0.8, 1.0])
+def test_context_and_batch_images_interchange(gpu_lib, tau_g, shape):
+    """FP32 images are one format whatever the population layout: a single
+    context's image resumes as a batch case (populations in planes) and a
+    batch case's image resumes in a single context, each bit for bit like
+    the run that never stopped."""
+    mcb = gpu_lib
+    p = params(mcb, mcb.STORE_FP32, tau_g, path=mcb.PATH_FUSED, shape=shape)
+    with mcb.Lattice(p) as a:
+        a.init_bubbles(BUB, 3.0)
+        a.step(6)
+        image = a.save_state()
+        a.step(7)
+        ref = snapshot(a)
+    with mcb.Batch(mcb.STORE_FP32) as bt:
+        other = bt.add_case(mcb.case_preset("batch", 3200), storage=mcb.STORE_FP32)
+        bt.step(1)                       # the resumed case joins at the other parity
+        cid = bt.add_state(image)
+        bt.step(7)
+        for name in ("f", "g"):
+            assert np.array_equal(bt.read(cid, name), ref[name]), f"context image in a batch: {name}"
+        back = bt.save_state(cid)
+        bt.remove(other)
+    with mcb.Lattice(p) as b:
+        b.load_state(back)
+        got = snapshot(b)
+    assert_same(dict(ref), got, "batch image in a context")
 
 
 def test_f_moments_image_continues_bit_for_bit(gpu_lib):

@@ -133,12 +133,17 @@ def test_f_moments_read_populations_and_refuse_writes(gpu_lib):
     lat.close()
 
 
+# FP32 contexts keep populations in planes at the first shape (nx*ny = 480)
+# and in 64-cell blocks at the second (nx*ny a multiple of 64).
+LAYOUT_SHAPES = [(24, 20, 16), (32, 24, 16)]
+
+
+@pytest.mark.parametrize("shape", LAYOUT_SHAPES)
 @pytest.mark.parametrize("normals", list(NORMALS))
-def test_moment_form_populations(gpu_lib, normals):
+def test_moment_form_populations(gpu_lib, normals, shape):
     """g read from the moment form is the reference layout, before and after
     steps; writing g switches the context to stored populations, bit for bit."""
     mcb = gpu_lib
-    shape = (24, 20, 16)
     p = mcb.make_params(*shape, tau_f=0.6, tau_g=1.0, sigma=0.08, path=mcb.PATH_FUSED, normals=NORMALS[normals])
     bubbles = [(12.0, 10.5, 8.0, 5.0)]
     o = oracle_for(p)
@@ -190,10 +195,10 @@ def test_fused_tilings_agree(gpu_lib, tau_g):
                         f"normals {normals}, f_form {f_form}, tiling {tx}x{ty} zc={zc}: {k} differs"
 
 
-def test_populations_roundtrip_and_layout(gpu_lib):
+@pytest.mark.parametrize("shape", LAYOUT_SHAPES)
+def test_populations_roundtrip_and_layout(gpu_lib, shape):
     """F/G readout is the reference layout (post-stream, pre-collision) for both paths."""
     mcb = gpu_lib
-    shape = (24, 20, 16)
     bubbles = [(12.0, 10.0, 8.0, 5.0)]
     outs = {}
     for path in ("fused", "staged"):
@@ -222,7 +227,9 @@ def test_populations_roundtrip_and_layout(gpu_lib):
         assert np.array_equal(outs["staged+1"][k], o.field(k))
 
 
-REF_SHAPES = [(24, 20, 40), (37, 29, 33)]   # the second has partial tiles in x and y
+# The second has partial tiles in x and y; the third keeps FP32 populations
+# in 64-cell blocks.
+REF_SHAPES = [(24, 20, 40), (37, 29, 33), (32, 24, 20)]
 
 
 @pytest.mark.parametrize("path", ["fused", "fused-n-recomputed", "staged"])
@@ -260,15 +267,16 @@ def test_reference_mode(gpu_lib, path, shape):
     lat.close()
 
 
+@pytest.mark.parametrize("shape", [(26, 22, 30), (32, 22, 30)])
 @pytest.mark.parametrize("normals", list(NORMALS))
-def test_reference_mode_fused_write_f_and_resume(gpu_lib, normals):
+def test_reference_mode_fused_write_f_and_resume(gpu_lib, normals, shape):
     """Fused ghost-layer path: F written through the ABI feeds the next
     iteration as the reference's d_f does (face populations included once),
     and a state image resumes the run bit for bit, also in a context that
     keeps the normals the other way; compared with the oracle given the
     same F."""
     mcb = gpu_lib
-    nx, ny, nz = 26, 22, 30
+    nx, ny, nz = shape
     p = mcb.make_params(nx, ny, nz, bc_mode=mcb.BC_REFERENCE, weights=mcb.WEIGHTS_LITERAL, tau_f=0.55,
                         tau_g=1.0, sigma=0.1, path=mcb.PATH_FUSED, normals=NORMALS[normals])
     bubbles = [(13.0, 11.0, 15.0, 6.5)]
@@ -305,8 +313,9 @@ def test_reference_mode_fused_write_f_and_resume(gpu_lib, normals):
     lat.close()
 
 
+@pytest.mark.parametrize("shape", [(30, 26, 22), (32, 26, 22)])
 @pytest.mark.parametrize("kind", ["moments", "moments-n-recomputed", "f-moments", "populations", "reference"])
-def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
+def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind, shape):
     """mcb_step replays pairs of fused steps from a captured hipGraph. Each
     replay, with eager work (observables, a field readout) and host
     synchronisation between replays, must give the bits of the same steps
@@ -321,7 +330,7 @@ def test_graph_replayed_steps_match_eager_steps(gpu_lib, kind):
         kw["normals"] = mcb.NORMALS_RECOMPUTED
     if kind == "f-moments":
         kw["f_form"] = mcb.F_MOMENTS
-    p = mcb.make_params(30, 26, 22, **kw)
+    p = mcb.make_params(*shape, **kw)
     bubbles = [(15.2, 13.1, 11.3, 6.0)]
     g, e = mcb.Lattice(p), mcb.Lattice(p)
     g.init_bubbles(bubbles)
