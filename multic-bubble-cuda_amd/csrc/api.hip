@@ -464,8 +464,8 @@ hipError_t pops_image_copy(mcb_ctx* c, bool to_image) {
     float* const* bufs[2] = {c->f, c->gm == GM_POPULATIONS ? c->g : nullptr};
     for (float* const* b : bufs) {
         if (!b) continue;
-        const hipError_t e = to_image ? launch_link_shift(b[cur], 1, b[idle], 0, c->geo, 0, c->stream)
-                                      : launch_link_shift(b[idle], 0, b[cur], 1, c->geo, 0, c->stream);
+        const hipError_t e = to_image ? launch_pop_relayout(b[cur], 1, b[idle], 0, c->geo, 4, c->stream)
+                                      : launch_pop_relayout(b[idle], 0, b[cur], 1, c->geo, 4, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -10,17 +10,18 @@
 namespace mcb {
 
 // Lattice geometry of one case: sides, cells, the population plane stride
-// and the layout of its FP32 population buffers.
+// and the layout of its FP32 population buffers (16-bit ones are always in
+// planes).
 //
 // Population layouts (cell index i = z*nxy + y*nx + x, link l):
 //   planes (blocked = 0): element l*plane + i, one plane of cells per link;
 //   blocks (blocked = 1): element (i >> 6)*19*64 + l*64 + (i & 63), groups of
-//     64 consecutive cells holding their 19 links side by side (1216 floats).
+//     64 consecutive cells holding their 19 links side by side.
 // Blocks need nxy to be a multiple of 64, so a z plane is a whole number of
-// groups and an offset splits into a wave-uniform z part (z*nxy*76 bytes) and
-// a per-lane in-plane part. A cell's 19 links then lie within 4864 bytes
-// instead of 19 planes apart: the collide pass's gathers and stores touch 19x
-// fewer DRAM pages per wave (256^3: 0.577 -> 0.544 ms per FP32 collide).
+// groups and an offset splits into a wave-uniform z part (z*nxy*19 elements)
+// and a per-lane in-plane part. A cell's 19 links then lie within 19*64
+// elements instead of 19 planes apart, and a wave's gathers and stores open
+// fewer DRAM pages (256^3: 0.577 -> 0.544 ms per FP32 collide).
 struct Geo {
     int nx, ny, nz;
     int nxy;          // nx*ny

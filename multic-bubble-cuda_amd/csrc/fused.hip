@@ -198,8 +198,7 @@ __global__ __launch_bounds__(PBX * PBY) void k_fm_to_ref(CaseView c, float* __re
     });
 }
 
-// dst_l(x) = src_l(x - SIGN c_l), src in layout sblk and dst in dblk (SIGN 0:
-// a change of layout).
+// dst_l(x) = src_l(x - SIGN c_l), src in layout sblk and dst in dblk.
 template <int SIGN>
 __global__ __launch_bounds__(PBX * PBY) void k_link_shift(const float* __restrict__ src, int sblk,
                                                           float* __restrict__ dst, int dblk, Geo G) {
@@ -215,6 +214,18 @@ __global__ __launch_bounds__(PBX * PBY) void k_link_shift(const float* __restric
         dst[pop_index(G, dblk, l, zs[1] + ys[1] + xs[1])] =
             src[pop_index(G, sblk, l, zs[1 - SIGN * CZ[l]] + ys[1 - SIGN * CY[l]] + xs[1 - SIGN * CX[l]])];
     });
+}
+
+// dst (layout dblk) = src (layout sblk): the 19 links of every cell, as raw
+// elements of T (4-byte FP32 or 2-byte 16-bit populations).
+template <class T>
+__global__ __launch_bounds__(256) void k_pop_relayout(const T* __restrict__ src, int sblk, T* __restrict__ dst,
+                                                      int dblk, Geo G) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < G.n; i += (long long)gridDim.x * 256)
+        for_links([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            dst[pop_index(G, dblk, l, i)] = src[pop_index(G, sblk, l, i)];
+        });
 }
 
 // --------------------------------------------------------------- reductions
@@ -498,8 +509,19 @@ hipError_t launch_link_shift(const float* src, int sblk, float* dst, int dblk, c
                              hipStream_t s) {
     const dim3 grid = cell_grid(geo), block(PBX, PBY, 1);
     if (sign > 0) hipLaunchKernelGGL(k_link_shift<1>, grid, block, 0, s, src, sblk, dst, dblk, geo);
-    else if (sign < 0) hipLaunchKernelGGL(k_link_shift<-1>, grid, block, 0, s, src, sblk, dst, dblk, geo);
-    else hipLaunchKernelGGL(k_link_shift<0>, grid, block, 0, s, src, sblk, dst, dblk, geo);
+    else hipLaunchKernelGGL(k_link_shift<-1>, grid, block, 0, s, src, sblk, dst, dblk, geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_pop_relayout(const void* src, int sblk, void* dst, int dblk, const Geo& geo, unsigned elem_bytes,
+                               hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<long long>((geo.n + 255) / 256, 256LL * 16);
+    if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_pop_relayout<unsigned>, dim3(blocks), dim3(256), 0, s, static_cast<const unsigned*>(src),
+                           sblk, static_cast<unsigned*>(dst), dblk, geo);
+    else
+        hipLaunchKernelGGL(k_pop_relayout<unsigned short>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<const unsigned short*>(src), sblk, static_cast<unsigned short*>(dst), dblk, geo);
     return hipGetLastError();
 }
 

@@ -178,11 +178,14 @@ hipError_t launch_fm_to_ref(const CaseView& c, float* dst, hipStream_t s);
 bool fm_tiling_supported(const Tiling& t);
 hipError_t launch_ab_to_ref(const CaseView& c, float* dst, hipStream_t s);
 // dst_l(x) = src_l(x - sign*c_l) for all 19 links (sign = +1: pull form ->
-// reference order; sign = -1: reference order -> pull form; 0: a copy), src
-// in population layout sblk and dst in dblk (Geo::blocked values; the
-// reference order is always in planes).
+// reference order; sign = -1: reference order -> pull form), src in
+// population layout sblk and dst in dblk (Geo::blocked values; the reference
+// order is always in planes).
 hipError_t launch_link_shift(const float* src, int sblk, float* dst, int dblk, const Geo& geo, int sign,
                              hipStream_t s);
+// The same populations in another layout (raw elements of elem_bytes, 4 or 2).
+hipError_t launch_pop_relayout(const void* src, int sblk, void* dst, int dblk, const Geo& geo, unsigned elem_bytes,
+                               hipStream_t s);
 
 // ---- batched (many cases, one launch per pass), any storage form
 // storage: MCB_STORE_FP32 / FP16 / BF16; work: device array of n_work items.
