@@ -24,9 +24,9 @@ namespace {
 
 constexpr int PBX = 64, PBY = 4;
 
-// Grids of the collide pass (FP32 populations in blocks) from which tiles run
-// in dispatch order instead of xcd_order: four workgroups per CU.
-constexpr unsigned kCollideNaturalOrder = 4 * 256;
+// Tiles per XCD run of the collide pass with FP32 populations in blocks
+// (xcd_runs).
+constexpr unsigned kCollideRun = 8;
 
 // 1-D grid of gx*gy*nz workgroups of 64x4 cells, renumbered by xcd_order.
 __global__ __launch_bounds__(PBX * PBY) void k_phase(CaseView c, unsigned gx, unsigned gy) {
@@ -120,15 +120,15 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) vo
 template <int TX, int TY, int MODE>
 __global__ __launch_bounds__(TX * TY) void k_collide(CaseView c) {
     const Geo& G = c.geo;
-    // 1-D grid of gx*gy*gz tiles, x fastest, renumbered by xcd_order, except
-    // for populations in 64-cell blocks on grids of kCollideNaturalOrder
-    // tiles or more: there the dispatch order (consecutive tiles on different
-    // XCDs, the whole device on one contiguous stretch of the blocks) beats
-    // the shared x-halo lines of the renumbering (collide at 256^3: 0.545 ->
-    // 0.521 ms, tau_g != 1 form 1.00 -> 0.94 ms; renumbered is faster by 1-4%
-    // at 96^3 and 128^3, within 1% at 192^3).
+    // 1-D grid of gx*gy*gz tiles, x fastest, renumbered by xcd_order, or by
+    // xcd_runs for populations in 64-cell blocks: with blocks, keeping the
+    // whole device on one contiguous stretch of the lattice matters more than
+    // the halo lines each XCD's own range shares (collide at 256^3: 0.545 ms
+    // renumbered by xcd_order, 0.521 in dispatch order, 0.517 in runs of 8;
+    // at 128^3, where every tile is resident at once, dispatch order was 4%
+    // slower than xcd_order and runs of 8 as fast).
     const unsigned gx = (unsigned)(G.nx + TX - 1) / TX, gy = (unsigned)(G.ny + TY - 1) / TY;
-    unsigned t = G.blocked && gridDim.x >= kCollideNaturalOrder ? blockIdx.x : xcd_order(blockIdx.x, gridDim.x);
+    unsigned t = G.blocked ? xcd_runs(blockIdx.x, gridDim.x, kCollideRun) : xcd_order(blockIdx.x, gridDim.x);
     const int x0 = (int)(t % gx) * TX;
     t /= gx;
     const int y0 = (int)(t % gy) * TY;

@@ -85,6 +85,17 @@ __host__ __device__ __forceinline__ unsigned xcd_order(unsigned lid, unsigned to
     const unsigned xcd = lid % kXcds, slot = lid / kXcds;
     return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
 }
+// A milder renumbering: the device keeps working on one contiguous stretch
+// of tiles, as in dispatch order, but each XCD takes runs of `run`
+// consecutive tiles of it (within every window of 8 * run workgroups, XCD j
+// takes tiles [window + j * run, window + (j + 1) * run)), so x-neighbouring
+// tiles of a run share their halo lines in one L2. A last partial window
+// keeps dispatch order. Bijective on [0, total).
+__host__ __device__ __forceinline__ unsigned xcd_runs(unsigned lid, unsigned total, unsigned run) {
+    const unsigned w = kXcds * run;
+    if (lid >= total / w * w) return lid;
+    return lid / w * w + (lid % kXcds) * run + (lid / kXcds) % run;
+}
 
 // ------------------------------------------------------------ storage forms
 
