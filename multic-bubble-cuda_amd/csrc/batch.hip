@@ -22,12 +22,12 @@ unsigned pop_bytes(int storage) { return storage == MCB_STORE_FP32 ? 4u : 2u; }
 
 constexpr int kMacroFields = 10;
 
-// Work items of a case (x fastest, then y, then z chunks), renumbered so that
-// x-neighbouring tiles meet in one XCD's L2: by xcd_order (each XCD walks a
-// contiguous range) for 16-bit batches, by xcd_runs (runs of 8 tiles, the
-// device on one contiguous stretch) for FP32 ones, whose populations are
-// mostly in 64-cell blocks (the 64 'batch' presets: 22.4k against 22.0k
-// MLUPS; 16-bit batches ran 2-7% slower without xcd_order).
+// Work items of a case (x fastest, then y, then z chunks), renumbered by
+// xcd_runs: each XCD takes runs of 8 consecutive tiles, so x-neighbouring
+// tiles meet in one L2 while the device works on one contiguous stretch of
+// the list (against xcd_order, where each XCD walks its own range: the 64
+// FP32 'batch' presets 22.0k -> 22.4k MLUPS, FP16 at 256^3 41.5k -> 42.7k,
+// 128^3 and 144^3 within 0.5%; without renumbering FP16 ran 2-7% slower).
 std::vector<WorkItem> tiles_of(const CaseState& c, int slot) {
     std::vector<WorkItem> v;
     const Tiling& t = c.tiling;
@@ -37,11 +37,10 @@ std::vector<WorkItem> tiles_of(const CaseState& c, int slot) {
     return v;
 }
 
-std::vector<WorkItem> xcd_permute(const std::vector<WorkItem>& in, int storage) {
+std::vector<WorkItem> xcd_permute(const std::vector<WorkItem>& in) {
     std::vector<WorkItem> out(in.size());
     const unsigned n = (unsigned)in.size();
-    for (unsigned lid = 0; lid < n; ++lid)
-        out[lid] = in[storage == MCB_STORE_FP32 ? xcd_runs(lid, n, 8) : xcd_order(lid, n)];
+    for (unsigned lid = 0; lid < n; ++lid) out[lid] = in[xcd_runs(lid, n, 8)];
     return out;
 }
 
@@ -379,7 +378,7 @@ mcb_status Batch::add_impl(const mcb_params& p, const float* phi_host, const Bub
 // The case's own work table (case index 0, used with the case's entry of the
 // view arrays as the base) and its partial-sum buffer, for its tiling.
 mcb_status Batch::build_own_table(CaseState* c) {
-    auto own = xcd_permute(tiles_of(*c, 0), c->storage);
+    auto own = xcd_permute(tiles_of(*c, 0));
     c->n_work = (unsigned)own.size();
     if (own.size() > c->work_cap) {
         if (c->d_work) B_HIP(hipFree(c->d_work));
@@ -486,7 +485,7 @@ mcb_status Batch::rebuild() {
         Group& gr = groups_[kv.first];
         gr.t = Tiling{std::get<0>(kv.first), std::get<1>(kv.first), 0};
         gr.mode = std::get<2>(kv.first);
-        staged.push_back(xcd_permute(kv.second, storage_));
+        staged.push_back(xcd_permute(kv.second));
         const auto& items = staged.back();
         if (items.size() > gr.cap) {
             if (gr.d_work) B_HIP(hipFree(gr.d_work));
