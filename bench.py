@@ -281,7 +281,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(whole cases or z-slabs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(whole cases or z-slabs would shard across GPUs; not built)", file=sys.stderr)
         sys.exit(2)
 
     import mcb

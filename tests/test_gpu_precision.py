@@ -3,7 +3,7 @@ an FP64 run of the oracle.
 
 The model is chaotic at the rounding level (a one-ulp change grows to
 percent-level differences in u over 1000 steps), so the per-field error of a
-reduced-precision run is judged against the FP32 transcription's own distance
+reduced-precision run is measured against the FP32 transcription's own distance
 from FP64 on the same case: the 16-bit forms may be at most RATIO times
 further from FP64 than FP32 is (plus a small floor). The physical observable,
 the Laplace pressure jump, must agree with FP64 to 1 %, and the phase must be
