@@ -111,6 +111,25 @@ def test_unsupported_combinations():
         assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED, kw
 
 
+def test_largest_fused_lattice():
+    """The fused kernels address a buffer's 19 population planes through one
+    32-bit buffer resource: FP32 lattices beyond 4 GiB of populations
+    (400^3: 64M cells x 19 x 4 B) are refused on the fused path before any
+    device is touched, and an unset path falls back to the staged kernels."""
+    L = mcb.lib()
+    h = ctypes.c_void_p()
+    p = mcb.make_params(400, 400, 400, path=mcb.PATH_FUSED)
+    assert L.mcb_create(ctypes.byref(p), None, ctypes.byref(h)) == mcb.ERR_UNSUPPORTED
+    assert not h.value
+    if mcb.device_count() == 0:
+        # Accepted shapes get as far as the device lookup: 352^3 (43.6M cells,
+        # 3.3 GB of f) on the fused path, 400^3 with the path left to the
+        # library (staged), and 400^3 with 16-bit populations (half the bytes).
+        for q in (mcb.make_params(352, 352, 352, path=mcb.PATH_FUSED), mcb.make_params(400, 400, 400),
+                  mcb.make_params(400, 400, 400, storage=mcb.STORE_FP16, path=mcb.PATH_FUSED)):
+            assert L.mcb_create(ctypes.byref(q), None, ctypes.byref(h)) == mcb.ERR_NO_DEVICE
+
+
 def test_footprint():
     n = 64 ** 3
     p = mcb.make_params(64, 64, 64, tau_g=0.8)
