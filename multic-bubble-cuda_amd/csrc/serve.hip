@@ -242,14 +242,13 @@ mcb_status serve_batched(const mcb_serve_config& cfg, const mcb_request* reqs, i
         if (ss != MCB_OK) return ss;
         st->rounds += 1;
         const Clock::time_point tr = Clock::now();
-        // Read out and release the cases that are done.
-        for (size_t j = 0; j < active.size();) {
-            Resident& a = active[j];
+        // Read out every case that is done, then release them all: a removal
+        // re-forms the batch (work lists, views, step graphs) before the next
+        // macro pass, so removing between read-outs would re-form it once per
+        // finished case instead of once per round.
+        for (Resident& a : active) {
             a.left -= k;
-            if (a.left > 0) {
-                ++j;
-                continue;
-            }
+            if (a.left > 0) continue;
             mcb_request_result& r = res[a.req];
             const mcb_request& q = reqs[a.req];
             mcb_status rs = batch.macros(a.id);
@@ -262,6 +261,13 @@ mcb_status serve_batched(const mcb_serve_config& cfg, const mcb_request* reqs, i
             r.lattice_updates = rs == MCB_OK ? c->geo.n * (long long)q.c.nsteps : 0;
             r.status = rs;
             r.finish_s = seconds_since(t0);
+        }
+        for (size_t j = 0; j < active.size();) {
+            const Resident& a = active[j];
+            if (a.left > 0) {
+                ++j;
+                continue;
+            }
             mcb_status rm = batch.remove(a.id);
             if (rm != MCB_OK) return rm;
             in_use -= a.bytes;
