@@ -22,6 +22,11 @@ unsigned pop_bytes(int storage) { return storage == MCB_STORE_FP32 ? 4u : 2u; }
 
 constexpr int kMacroFields = 10;
 
+// Batches of at least this many cells launch their steps one by one instead
+// of replaying captured step graphs (see rebuild): a step of 8M cells takes
+// ~0.2 ms, several times what enqueueing its launches takes.
+constexpr long long kCellsEager = 8LL << 20;
+
 // Work items of a case (x fastest, then y, then z chunks), renumbered by
 // xcd_runs: each XCD takes runs of 8 consecutive tiles, so x-neighbouring
 // tiles meet in one L2 while the device works on one contiguous stretch of
@@ -503,7 +508,13 @@ mcb_status Batch::rebuild() {
         if (g) B_HIP(hipGraphExecDestroy(g));
         g = nullptr;
     }
-    if (use_graphs_ && n) {
+    // Step graphs pay where a step is short next to its launches (a few small
+    // cases); a batch of kCellsEager cells or more keeps the device busy with
+    // launches enqueued one by one, and a serving loop that re-forms it every
+    // round would only pay the capture.
+    long long cells = 0;
+    for (const auto& kv : cases_) cells += kv.second->geo.n;
+    if (use_graphs_ && n && cells < kCellsEager) {
         for (int q = 0; q < 2; ++q) {
             hipGraph_t gr = nullptr;
             if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
