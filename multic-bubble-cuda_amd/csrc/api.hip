@@ -425,8 +425,9 @@ mcb_status any_step(mcb_ctx* c, int nsteps) {
 // reference variable (src/var.cu:13-18): with ghost-layer boundaries the
 // kernels leave the face values of phi and the macroscopic fields from one
 // iteration to the next, so they are state too; fused FP32, f of the current
-// buffer and g as populations or in moment form (the state's phi and u; n
-// is gradCalc of that phi, rebuilt on loading where the context stores it).
+// buffer and g as populations (with the phi the last step reported) or in
+// moment form (the state's phi and u; n is gradCalc of that phi, rebuilt on
+// loading where the context stores it).
 std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     if (c->path == MCB_PATH_STAGED) {
@@ -442,11 +443,15 @@ std::vector<StatePart> ctx_state_parts(mcb_ctx* c) {
     if (c->fm)   // the 13 moments (u among them) and the state's phi
         return {{"fm", c->f[c->cur], kFm, N, P, 4, false}, {"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false}};
     // Images keep populations in planes: with blocked populations they pass
-    // through the idle A/B buffers (pops_to_image / pops_from_image).
+    // through the idle A/B buffers (pops_image_copy / pops_image_done).
     const int side = c->geo.blocked ? c->cur ^ 1 : c->cur;
     std::vector<StatePart> parts{{"f", c->f[side], 19, N, P, 4, false}};
     if (c->gm == GM_POPULATIONS) {
+        // With g as populations phi is what the last step reported (phiCalc
+        // of the populations before that step's collision): not given back
+        // by the state's populations, so kept for a read after loading.
         parts.push_back({"g", c->g[side], 19, N, P, 4, false});
+        parts.push_back({"phi", c->phi, 1, N, N, 4, false});
     } else {
         static const char* const un[3] = {"u0", "u1", "u2"};
         parts.push_back({"phi", c->phis[c->cur ^ 1], 1, N, N, 4, false});

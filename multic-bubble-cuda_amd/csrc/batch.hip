@@ -693,11 +693,12 @@ mcb_status Batch::to_populations(CaseState* c) {
 
 // What the next step of a case reads (view(cur)): f of the current buffer;
 // g as populations, or its moment form (u in FP32; B in 16 bits); the
-// state's phi where a step reads it (the moment form's phi, the 16-bit g
-// anchor; FP32 g populations alone give phi); the 16-bit mass target and the
-// correction pending for the next phase pass. n is not part of an image: it
-// is gradCalc of the state's phi, rebuilt on loading. FP32 images have the
-// parts of a single context's (same form), so either resumes in the other.
+// state's phi (the moment form's phi, the 16-bit g anchor; with FP32 g
+// populations the phi the last step reported, which the populations do not
+// give back); the 16-bit mass target and the correction pending for the next
+// phase pass. n is not part of an image: it is gradCalc of the state's phi,
+// rebuilt on loading. FP32 images have the parts of a single context's (same
+// form), so either resumes in the other.
 std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     const size_t N = (size_t)c->geo.n, P = (size_t)c->geo.plane;
     const unsigned eb = pop_bytes(c->storage);
@@ -707,8 +708,7 @@ std::vector<StatePart> Batch::state_parts(CaseState* c, double* mass) {
     std::vector<StatePart> parts;
     parts.push_back({"f", c->f[side], 19, N, P, eb, false});
     if (c->gm == GM_POPULATIONS) parts.push_back({"g", c->g[side], 19, N, P, eb, false});
-    if (c->storage != MCB_STORE_FP32 || c->gm != GM_POPULATIONS)
-        parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
+    parts.push_back({"phi", c->phi[c->cur ^ 1], 1, N, N, 4, false});
     if (c->gm != GM_POPULATIONS) {
         static const char* const un[3] = {"u0", "u1", "u2"};
         for (int k = 0; k < 3; ++k)

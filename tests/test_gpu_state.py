@@ -59,14 +59,17 @@ def test_save_load_continues_bit_for_bit(gpu_lib, form, k, shape):
     a.init_bubbles(BUB, 3.0)
     a.step(k)
     image = a.save_state()
+    phi_saved = a.read("phi")
     pi, steps = mcb.state_info(image)
     assert steps == k and (pi.nx, pi.ny, pi.nz) == shape
     a.step(9)
     ref = snapshot(a)
     a.close()
-    # A fresh context (no initial state of its own) resumes from the image.
+    # A fresh context (no initial state of its own) resumes from the image;
+    # phi read before any step is the phi the saved context reported.
     b = mcb.Lattice(p)
     b.load_state(image)
+    assert np.array_equal(b.read("phi"), phi_saved), f"{form} k={k}: phi read after loading"
     b.step(9)
     assert_same(ref, snapshot(b), f"{form} k={k}")
     # Loading into a context that has stepped an odd number of times (the
