@@ -159,7 +159,7 @@ Tiling choose_tiling_single(const mcb_params& p) {
     // Where the chunks are already down to 4 planes, 4-row tiles give twice
     // the workgroups for the same halo restaging per plane (the reference
     // preset: 64x4x4 0.0591-0.0595 ms per step against 0.0610-0.0613 with
-    // 64x8x4).
+    // 64x8x4; 64^3: 0.0317 against 0.0363 ms).
     if (t.zc <= 4 && t.tx == 64 && t.ty == 8 && p.storage == MCB_STORE_FP32) t.ty = 4;
     return t;
 }
